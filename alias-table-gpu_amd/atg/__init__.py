@@ -1,0 +1,167 @@
+"""Python view of the atg C ABI (include/atg.h) for tests and the benchmark.
+
+`Table` wraps one alias table (AliasTableSplitGpu + its samplers and the
+rejection structure of the reference); device buffers are torch tensors or raw
+device pointers.  Every call goes through libatg.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ATG_AOS, ATG_F32, ATG_F64, ATG_MEM_DEVICE, ATG_MEM_HOST, ATG_SAMPLE_BASIC,
+                   ATG_SAMPLE_SECTIONED, ATG_SOA, ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED, DIST, AtgError,
+                   BuildOpts, Stats, check, lib)
+
+__all__ = ["Table", "generate_weights", "default_opts", "lib", "AtgError", "DIST"]
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return C.c_void_p(a)
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if hasattr(s, "cuda_stream"):
+        return C.c_void_p(s.cuda_stream)
+    return C.c_void_p(int(s))
+
+
+def default_opts():
+    o = BuildOpts()
+    lib().atg_default_opts(C.byref(o))
+    return o
+
+
+def generate_weights(dist, n, seed=42, param=0.0):
+    """Host FP64 weights (atg_generate_weights)."""
+    w = np.empty(int(n), dtype=np.float64)
+    d = DIST[dist] if isinstance(dist, str) else int(dist)
+    check(lib().atg_generate_weights(d, int(n), int(seed), float(param), _ptr(w)))
+    return w
+
+
+class Table:
+    """One alias table on the GPU (AliasTableSplitGpu<TableStorage> of the reference)."""
+
+    def __init__(self, n, dtype="f64", layout="aos", psa_plus=True, chains=16, section_size=0, num_splits=0,
+                 split="bisect", learned_segments=4096, record_phases=False):
+        o = default_opts()
+        o.psa_plus = 1 if psa_plus else 0
+        o.chains = int(chains)
+        o.section_size = int(section_size)
+        o.num_splits = int(num_splits)
+        o.split_method = ATG_SPLIT_LEARNED if split == "learned" else ATG_SPLIT_BISECT
+        o.learned_segments = int(learned_segments)
+        o.record_phases = 1 if record_phases else 0
+        self.n = int(n)
+        self.dtype = dtype
+        self.layout = layout
+        h = C.c_void_p()
+        check(lib().atg_create(C.byref(h), self.n, ATG_F32 if dtype == "f32" else ATG_F64,
+                               ATG_SOA if layout == "soa" else ATG_AOS, C.byref(o)))
+        self.h = h
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            lib().atg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- construction ---------------------------------------------------
+    def set_weights(self, w, stream=None):
+        if hasattr(w, "is_cuda") and w.is_cuda:
+            self._keep = w
+            check(lib().atg_set_weights(self.h, _ptr(w), ATG_MEM_DEVICE, _stream(stream)))
+        else:
+            want = np.float32 if self.dtype == "f32" else np.float64
+            a = np.ascontiguousarray(w, dtype=want)
+            assert a.size == self.n
+            check(lib().atg_set_weights(self.h, _ptr(a), ATG_MEM_HOST, _stream(stream)))
+
+    def build(self, stream=None):
+        check(lib().atg_build(self.h, _stream(stream)))
+
+    def sync(self, stream=None):
+        check(lib().atg_sync(self.h, _stream(stream)))
+
+    def stats(self):
+        st = Stats()
+        check(lib().atg_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def post_build(self, verify=True):
+        rel = C.c_double(0.0)
+        ok = C.c_int32(0)
+        check(lib().atg_post_build(self.h, 1 if verify else 0, C.byref(rel), C.byref(ok)))
+        return rel.value, bool(ok.value)
+
+    def table(self):
+        w = np.empty(self.n, dtype=np.float64)
+        a = np.empty(self.n, dtype=np.int32)
+        check(lib().atg_get_table(self.h, _ptr(w), _ptr(a), ATG_MEM_HOST, None))
+        return w, a
+
+    def device_table_ptr(self):
+        p0, p1 = C.c_void_p(), C.c_void_p()
+        check(lib().atg_table_device_ptr(self.h, C.byref(p0), C.byref(p1)))
+        return p0.value, p1.value
+
+    def splits(self):
+        p = self.stats()["p"]
+        i = np.empty(p + 1, np.int32)
+        j = np.empty(p + 1, np.int32)
+        s = np.empty(p + 1, np.float64)
+        check(lib().atg_get_splits(self.h, _ptr(i), _ptr(j), _ptr(s), p + 1))
+        return i, j, s
+
+    def lh(self):
+        st = self.stats()
+        nl, nh = st["numL"], st["numH"]
+        out = {"L_item": np.empty(nl, np.int32), "L_w": np.empty(nl), "L_pref": np.empty(nl),
+               "H_item": np.empty(nh, np.int32), "H_w": np.empty(nh), "H_pref": np.empty(nh)}
+        check(lib().atg_get_lh(self.h, *[_ptr(out[k]) for k in
+                                         ("L_item", "L_w", "L_pref", "H_item", "H_w", "H_pref")]))
+        return out
+
+    # -- sampling ----------------------------------------------------------
+    def sample(self, count, seed, kind="sectioned", section_rows=4096, out=None, hist=None, checksum=False,
+               stream=None):
+        k = ATG_SAMPLE_SECTIONED if kind == "sectioned" else ATG_SAMPLE_BASIC
+        cs = C.c_uint64(0)
+        check(lib().atg_sample(self.h, k, int(count), int(seed), int(section_rows), _ptr(out), _ptr(hist),
+                               C.byref(cs) if checksum else None, _stream(stream)))
+        return cs.value if checksum else None
+
+    def rejection_build(self, stream=None):
+        check(lib().atg_rejection_build(self.h, _stream(stream)))
+
+    def rejection_size(self):
+        v = C.c_uint64(0)
+        check(lib().atg_rejection_size(self.h, C.byref(v)))
+        return v.value
+
+    def rejection_array(self):
+        sz = self.rejection_size()
+        A = np.empty(sz, np.int32)
+        check(lib().atg_get_rejection_array(self.h, _ptr(A), sz))
+        return A
+
+    def rejection_sample(self, count, seed, out=None, hist=None, checksum=False, stream=None):
+        cs = C.c_uint64(0)
+        check(lib().atg_rejection_sample(self.h, int(count), int(seed), _ptr(out), _ptr(hist),
+                                         C.byref(cs) if checksum else None, _stream(stream)))
+        return cs.value if checksum else None

@@ -1,0 +1,567 @@
+// C ABI of the engine (include/atg.h): handle lifetime, device memory, the
+// build/sample orchestration on the caller's stream, copy-out and the table
+// check of postBuild/verifyTableCorrectness.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/atg.h"
+#include "atg_build.h"
+#include "atg_sample.h"
+
+using namespace atg;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+atg_status fail(atg_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return st;
+}
+
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? ATG_ENOMEM : ATG_EDEVICE, "%s: %s (%s:%d)", \
+                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                   \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+int device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 256;
+    return cus;
+}
+
+constexpr int64_t kMaxSectionLen = 15360;  // LDS of the partition kernel: 10 B per item
+constexpr int64_t kMaxN = 0x7FFFFFF0LL;
+
+}  // namespace
+
+struct atg_table_s {
+    int64_t n = 0;
+    atg_dtype dtype = ATG_F64;
+    atg_layout layout = ATG_AOS;
+    atg_build_opts opts{};
+    int64_t G = 0, Ls = 0, p_req = 0, tree_half = 0;
+    int device = 0;
+    // device memory
+    void* d_weights_owned = nullptr;
+    const void* d_weights = nullptr;
+    DevState* d_state = nullptr;
+    double* d_section_sums = nullptr;
+    double* d_tree = nullptr;
+    uint64_t* d_slots = nullptr;
+    int32_t *d_Li = nullptr, *d_Hi = nullptr;
+    double *d_Lw = nullptr, *d_Lp = nullptr, *d_Hw = nullptr, *d_Hp = nullptr;
+    Row* d_rows = nullptr;
+    double* d_tw = nullptr;
+    int32_t* d_ta = nullptr;
+    Split* d_splits = nullptr;
+    int32_t* d_knots = nullptr;
+    unsigned long long* d_checksum = nullptr;
+    // rejection
+    int32_t* d_A = nullptr;
+    uint64_t A_cap = 0;
+    uint64_t *d_rej_tot = nullptr, *d_rej_off = nullptr;
+    bool rej_built = false;
+    // host side
+    std::vector<double> h_weights;  // host copy when weights were given on the host
+    bool weights_set = false, built = false;
+    hipEvent_t ev[6] = {};
+    bool events = false;
+
+    BuildArgs args() const {
+        BuildArgs a{};
+        a.weights = d_weights;
+        a.dtype_f32 = dtype == ATG_F32;
+        a.n = n;
+        a.G = G;
+        a.Ls = Ls;
+        a.Ls_max = Ls;
+        a.st = d_state;
+        a.section_sums = d_section_sums;
+        a.tree_scratch = d_tree;
+        a.tree_scratch_half = tree_half;
+        a.slots = d_slots;
+        a.Li = d_Li; a.Lw = d_Lw; a.Lp = d_Lp;
+        a.Hi = d_Hi; a.Hw = d_Hw; a.Hp = d_Hp;
+        a.rows = d_rows;
+        a.tw = d_tw;
+        a.ta = d_ta;
+        a.psa_plus = opts.psa_plus;
+        a.chains = opts.chains;
+        a.p_req = p_req;
+        a.splits = d_splits;
+        a.learned_segments = opts.learned_segments;
+        a.knot_j = d_knots;
+        return a;
+    }
+};
+
+extern "C" {
+
+void atg_default_opts(atg_build_opts* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->psa_plus = 1;
+    o->chains = 16;         // NUM_LOCAL_SPLITS (PsaPlusNormal.cu:3)
+    o->section_size = 0;    // min(1000, N/10) (AliasTableSplitGpu.cu:356)
+    o->num_splits = 0;      // AUTO (AliasTableSplitGpu.cu:58-70)
+    o->split_method = ATG_SPLIT_BISECT;
+    o->learned_segments = 4096;
+    o->record_phases = 0;
+}
+
+const char* atg_last_error(void) { return g_last_error.c_str(); }
+const char* atg_version(void) { return "atg 0.1 (gfx950)"; }
+int32_t atg_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+static void free_table(atg_table* t) {
+    if (!t) return;
+    void* ptrs[] = {t->d_weights_owned, t->d_state, t->d_section_sums, t->d_tree, t->d_slots, t->d_Li, t->d_Hi,
+                    t->d_Lw, t->d_Lp, t->d_Hw, t->d_Hp, t->d_rows, t->d_tw, t->d_ta, t->d_splits, t->d_knots,
+                    t->d_checksum, t->d_A, t->d_rej_tot, t->d_rej_off};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (t->events)
+        for (auto& e : t->ev) (void)hipEventDestroy(e);
+}
+
+void atg_destroy(atg_table* t) {
+    free_table(t);
+    delete t;
+}
+
+atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout layout, const atg_build_opts* opts) {
+    if (!out) return fail(ATG_EINVAL, "atg_create: out is NULL");
+    *out = nullptr;
+    if (n < 1 || n > kMaxN) return fail(ATG_EINVAL, "atg_create: n=%lld outside [1, %lld]", (long long)n, (long long)kMaxN);
+    if (dtype != ATG_F64 && dtype != ATG_F32) return fail(ATG_EINVAL, "atg_create: bad dtype");
+    if (layout != ATG_AOS && layout != ATG_SOA) return fail(ATG_EINVAL, "atg_create: bad layout");
+    atg_build_opts o;
+    if (opts) o = *opts; else atg_default_opts(&o);
+    if (o.chains < 1 || o.chains > 64) return fail(ATG_EINVAL, "atg_create: chains=%d outside [1, 64]", o.chains);
+    if (o.split_method != ATG_SPLIT_BISECT && o.split_method != ATG_SPLIT_LEARNED)
+        return fail(ATG_EINVAL, "atg_create: unknown split method %d", o.split_method);
+    if (o.split_method == ATG_SPLIT_LEARNED)
+        return fail(ATG_ENOTSUP, "atg_create: learned split index not available in this build");
+    if (o.learned_segments <= 0) o.learned_segments = 4096;
+    if (o.section_size <= 0) o.section_size = std::min<int64_t>(1000, n / 10);
+    if (o.section_size < 1) o.section_size = 1;
+    if (o.num_splits <= 0) o.num_splits = 32LL * device_cus() * 10;
+
+    atg_table* t = new atg_table_s();
+    t->n = n;
+    t->dtype = dtype;
+    t->layout = layout;
+    t->opts = o;
+    section_layout(n, o.section_size, &t->G, &t->Ls);
+    if (t->Ls > kMaxSectionLen) {
+        delete t;
+        return fail(ATG_EINVAL, "atg_create: section length %lld exceeds %lld", (long long)t->Ls, (long long)kMaxSectionLen);
+    }
+    t->p_req = std::max<int64_t>(2, std::min<int64_t>(o.num_splits, n / 10));
+    t->tree_half = (t->G + 63) / 64 + 1;
+    (void)hipGetDevice(&t->device);
+
+    auto cleanup = [&](atg_status st) { atg_destroy(t); return st; };
+#define ALLOC(ptr, cnt)                                                                          \
+    do {                                                                                         \
+        if (dalloc(&(ptr), (size_t)(cnt)) != hipSuccess) {                                       \
+            (void)hipGetLastError();                                                             \
+            return cleanup(fail(ATG_ENOMEM, "atg_create: hipMalloc of %s (%lld items) failed", #ptr, (long long)(cnt))); \
+        }                                                                                        \
+    } while (0)
+    ALLOC(t->d_state, 1);
+    ALLOC(t->d_section_sums, t->G);
+    ALLOC(t->d_tree, 2 * t->tree_half);
+    ALLOC(t->d_slots, 6 * t->G);
+    ALLOC(t->d_Li, n + 2);
+    ALLOC(t->d_Hi, n + 2);
+    ALLOC(t->d_Lw, n + 2);
+    ALLOC(t->d_Lp, n + 2);
+    ALLOC(t->d_Hw, n + 2);
+    ALLOC(t->d_Hp, n + 2);
+    if (layout == ATG_AOS) {
+        ALLOC(t->d_rows, n);
+    } else {
+        ALLOC(t->d_tw, n);
+        ALLOC(t->d_ta, n);
+    }
+    ALLOC(t->d_splits, t->p_req + 1);
+    ALLOC(t->d_knots, o.learned_segments + 2);
+    ALLOC(t->d_checksum, 1);
+#undef ALLOC
+    if (hipMemset(t->d_state, 0, sizeof(DevState)) != hipSuccess) return cleanup(fail(ATG_EDEVICE, "memset"));
+    if (set_partition_lds_limit(t->Ls) != hipSuccess) return cleanup(fail(ATG_EDEVICE, "partition LDS limit"));
+    if (o.record_phases) {
+        for (auto& e : t->ev)
+            if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(ATG_EDEVICE, "hipEventCreate"));
+        t->events = true;
+    }
+    *out = t;
+    return ATG_OK;
+}
+
+atg_status atg_set_weights(atg_table* t, const void* w, atg_mem mem, void* stream) {
+    if (!t || !w) return fail(ATG_EINVAL, "atg_set_weights: NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t bytes = (size_t)t->n * (t->dtype == ATG_F32 ? 4 : 8);
+    if (mem == ATG_MEM_DEVICE) {
+        t->d_weights = w;
+        t->h_weights.clear();
+    } else if (mem == ATG_MEM_HOST) {
+        if (!t->d_weights_owned) HIPCHK(hipMalloc(&t->d_weights_owned, bytes));
+        HIPCHK(hipMemcpyAsync(t->d_weights_owned, w, bytes, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        t->d_weights = t->d_weights_owned;
+        t->h_weights.resize(t->n);
+        if (t->dtype == ATG_F32) {
+            const float* f = (const float*)w;
+            for (int64_t i = 0; i < t->n; ++i) t->h_weights[i] = (double)f[i];
+        } else {
+            std::memcpy(t->h_weights.data(), w, bytes);
+        }
+    } else {
+        return fail(ATG_EINVAL, "atg_set_weights: bad mem kind");
+    }
+    t->weights_set = true;
+    t->built = false;
+    t->rej_built = false;
+    return ATG_OK;
+}
+
+atg_status atg_build(atg_table* t, void* stream) {
+    if (!t) return fail(ATG_EINVAL, "atg_build: NULL table");
+    if (!t->weights_set) return fail(ATG_ESTATE, "atg_build: weights not set");
+    hipStream_t s = (hipStream_t)stream;
+    const BuildArgs a = t->args();
+    if (t->events) HIPCHK(hipEventRecord(t->ev[0], s));
+    HIPCHK(launch_sum(a, s));
+    if (t->events) HIPCHK(hipEventRecord(t->ev[1], s));
+    HIPCHK(launch_partition(a, s));
+    if (t->events) HIPCHK(hipEventRecord(t->ev[2], s));
+    if (t->events) HIPCHK(hipEventRecord(t->ev[3], s));
+    HIPCHK(launch_split(a, s));
+    if (t->events) HIPCHK(hipEventRecord(t->ev[4], s));
+    HIPCHK(launch_pack(a, s));
+    if (t->events) HIPCHK(hipEventRecord(t->ev[5], s));
+    t->built = true;
+    return ATG_OK;
+}
+
+static atg_status state_status(const DevState& st) {
+    if (st.error & ERR_BAD_WEIGHT) return fail(ATG_EINVAL, "weights contain NaN, infinity or negative values");
+    if (st.error & ERR_BAD_SUM) return fail(ATG_EINVAL, "sum of weights is not positive and finite");
+    if (st.error & ERR_SECTION_TOO_LONG) return fail(ATG_EINVAL, "section too long for LDS");
+    if (st.error & ERR_LOOKBACK_TIMEOUT) return fail(ATG_ETIMEOUT, "look-back wait expired");
+    return ATG_OK;
+}
+
+atg_status atg_sync(atg_table* t, void* stream) {
+    if (!t) return fail(ATG_EINVAL, "atg_sync: NULL table");
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    DevState st;
+    HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
+    return state_status(st);
+}
+
+atg_status atg_get_stats(atg_table* t, atg_stats* out) {
+    if (!t || !out) return fail(ATG_EINVAL, "atg_get_stats: NULL argument");
+    HIPCHK(hipDeviceSynchronize());
+    DevState st;
+    HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->W = st.W;
+    out->W_N = st.W_N;
+    out->n = t->n;
+    out->numL = st.numL;
+    out->numH = st.numH;
+    out->R = st.numL + st.numH - 2;
+    out->p = st.p;
+    out->num_sections = t->G;
+    out->section_len = t->Ls;
+    out->split_fallbacks = (int64_t)st.split_fallbacks;
+    out->learned_fallbacks = (int64_t)st.learned_fallbacks;
+    out->degenerate = (int32_t)st.degenerate;
+    double mdl, mdh;
+    std::memcpy(&mdl, &st.cert_max_dL, 8);
+    std::memcpy(&mdh, &st.cert_min_dH, 8);
+    out->monotone = mdl < mdh ? 1 : 0;
+    if (t->events && t->built) {
+        float ms;
+        const int map[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}};
+        for (int k = 0; k < 5; ++k) {
+            if (hipEventElapsedTime(&ms, t->ev[map[k][0]], t->ev[map[k][1]]) == hipSuccess) out->phase_ms[k] = ms;
+        }
+    }
+    return state_status(st);
+}
+
+atg_status atg_get_table(atg_table* t, double* w, int32_t* al, atg_mem mem, void* stream) {
+    if (!t || !w || !al) return fail(ATG_EINVAL, "atg_get_table: NULL argument");
+    if (!t->built) return fail(ATG_ESTATE, "atg_get_table: not built");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = (size_t)t->n;
+    if (t->layout == ATG_SOA) {
+        const hipMemcpyKind k = mem == ATG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        HIPCHK(hipMemcpyAsync(w, t->d_tw, n * 8, k, s));
+        HIPCHK(hipMemcpyAsync(al, t->d_ta, n * 4, k, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return ATG_OK;
+    }
+    // AoS: strided copies split the 16-byte rows.
+    const hipMemcpyKind k = mem == ATG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIPCHK(hipMemcpy2DAsync(w, 8, &t->d_rows[0].weight, sizeof(Row), 8, n, k, s));
+    HIPCHK(hipMemcpy2DAsync(al, 4, &t->d_rows[0].alias, sizeof(Row), 4, n, k, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return ATG_OK;
+}
+
+atg_status atg_table_device_ptr(atg_table* t, void** a, void** b) {
+    if (!t || !a) return fail(ATG_EINVAL, "atg_table_device_ptr: NULL argument");
+    if (t->layout == ATG_AOS) {
+        *a = t->d_rows;
+        if (b) *b = nullptr;
+    } else {
+        *a = t->d_tw;
+        if (b) *b = t->d_ta;
+    }
+    return ATG_OK;
+}
+
+atg_status atg_get_splits(atg_table* t, int32_t* i, int32_t* j, double* spill, int64_t cap) {
+    if (!t || !i || !j || !spill) return fail(ATG_EINVAL, "atg_get_splits: NULL argument");
+    DevState st;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
+    if (st.p + 1 > cap) return fail(ATG_EINVAL, "atg_get_splits: capacity %lld < p+1=%lld", (long long)cap, (long long)(st.p + 1));
+    std::vector<Split> sp(st.p + 1);
+    if (st.p > 0) HIPCHK(hipMemcpy(sp.data(), t->d_splits, sizeof(Split) * (st.p + 1), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k <= st.p && st.p > 0; ++k) {
+        i[k] = sp[k].i;
+        j[k] = sp[k].j;
+        spill[k] = sp[k].spill;
+    }
+    return ATG_OK;
+}
+
+atg_status atg_get_lh(atg_table* t, int32_t* Li, double* Lw, double* Lp, int32_t* Hi, double* Hw, double* Hp) {
+    if (!t) return fail(ATG_EINVAL, "atg_get_lh: NULL table");
+    DevState st;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
+    if (Li) HIPCHK(hipMemcpy(Li, t->d_Li, 4 * st.numL, hipMemcpyDeviceToHost));
+    if (Lw) HIPCHK(hipMemcpy(Lw, t->d_Lw, 8 * st.numL, hipMemcpyDeviceToHost));
+    if (Lp) HIPCHK(hipMemcpy(Lp, t->d_Lp, 8 * st.numL, hipMemcpyDeviceToHost));
+    if (Hi) HIPCHK(hipMemcpy(Hi, t->d_Hi, 4 * st.numH, hipMemcpyDeviceToHost));
+    if (Hw) HIPCHK(hipMemcpy(Hw, t->d_Hw, 8 * st.numH, hipMemcpyDeviceToHost));
+    if (Hp) HIPCHK(hipMemcpy(Hp, t->d_Hp, 8 * st.numH, hipMemcpyDeviceToHost));
+    return ATG_OK;
+}
+
+// postBuild + verifyTableCorrectness (AliasTable.cu:24-65).  The implied mass
+// of item j is min(w_j, W/N) + sum over rows aliased to j of (W/N - row.w),
+// accumulated in binary128; compared as probabilities with the weights.
+atg_status atg_post_build(atg_table* t, int32_t verify, double* max_rel_err, int32_t* ok) {
+    if (!t) return fail(ATG_EINVAL, "atg_post_build: NULL table");
+    atg_status rc = atg_sync(t, nullptr);
+    if (rc != ATG_OK) return rc;
+    if (!verify) {
+        if (ok) *ok = 1;
+        return ATG_OK;
+    }
+    const int64_t n = t->n;
+    std::vector<double> rw(n), wt;
+    std::vector<int32_t> ra(n);
+    rc = atg_get_table(t, rw.data(), ra.data(), ATG_MEM_HOST, nullptr);
+    if (rc != ATG_OK) return rc;
+    const double* w = t->h_weights.data();
+    if (t->h_weights.empty()) {
+        wt.resize(n);
+        if (t->dtype == ATG_F32) {
+            std::vector<float> f(n);
+            HIPCHK(hipMemcpy(f.data(), t->d_weights, 4 * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) wt[i] = f[i];
+        } else {
+            HIPCHK(hipMemcpy(wt.data(), t->d_weights, 8 * n, hipMemcpyDeviceToHost));
+        }
+        w = wt.data();
+    }
+    DevState st;
+    HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
+    std::vector<__float128> m(n, 0);
+    const double W_N = st.W_N;
+    for (int64_t i = 0; i < n; ++i) {
+        if (rw[i] < W_N) {
+            m[i] += (__float128)rw[i];
+            const int32_t a = ra[i];
+            const __float128 rest = (__float128)W_N - (__float128)rw[i];
+            if (a >= 0 && a < n) m[a] += rest; else m[i] += rest;
+        } else {
+            m[i] += (__float128)W_N;
+        }
+    }
+    const __float128 total = (__float128)W_N * (__float128)n;
+    double mrel = 0;
+    int good = 1;
+    const double maxError = std::max((double)n * 1e-8, 1e-3);
+    for (int64_t i = 0; i < n; ++i) {
+        const __float128 pi = m[i] / total, pt = (__float128)w[i] / (__float128)st.W;
+        __float128 d = pi - pt;
+        if (d < 0) d = -d;
+        const double rel = w[i] > 0 ? (double)(d / pt) : (double)d;
+        if (rel > mrel) mrel = rel;
+        __float128 dm = m[i] - (__float128)w[i];
+        if (dm < 0) dm = -dm;
+        if ((double)dm > maxError || rw[i] < 0 || rw[i] > W_N * (1.0 + maxError)) good = 0;
+    }
+    if (max_rel_err) *max_rel_err = mrel;
+    if (ok) *ok = good;
+    return ATG_OK;
+}
+
+atg_status atg_sample(atg_table* t, atg_sample_kind kind, uint64_t count, uint64_t seed, int64_t section_rows,
+                      int32_t* out, uint32_t* hist, uint64_t* checksum, void* stream) {
+    if (!t) return fail(ATG_EINVAL, "atg_sample: NULL table");
+    if (!t->built) return fail(ATG_ESTATE, "atg_sample: table not built");
+    hipStream_t s = (hipStream_t)stream;
+    SampleArgs a{};
+    a.rows = t->d_rows;
+    a.tw = t->d_tw;
+    a.ta = t->d_ta;
+    a.st = t->d_state;
+    a.n = t->n;
+    a.count = count;
+    a.seed = seed;
+    a.out = out;
+    a.hist = hist;
+    a.checksum = checksum ? t->d_checksum : nullptr;
+    if (checksum) HIPCHK(hipMemsetAsync(t->d_checksum, 0, 8, s));
+    if (count > 0) {
+        if (kind == ATG_SAMPLE_BASIC) {
+            HIPCHK(launch_sample_basic(a, s));
+        } else if (kind == ATG_SAMPLE_SECTIONED) {
+            if (section_rows <= 0) section_rows = 4096;
+            if (section_rows > 8192) return fail(ATG_EINVAL, "atg_sample: section_rows %lld > 8192", (long long)section_rows);
+            a.SS = section_rows;
+            a.nsec = (t->n + section_rows - 1) / section_rows;
+            HIPCHK(set_sample_lds_limit((int)(section_rows * sizeof(Row))));
+            HIPCHK(launch_sample_sectioned(a, s));
+        } else {
+            return fail(ATG_EINVAL, "atg_sample: unknown kind %d", (int)kind);
+        }
+    }
+    if (checksum) {
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpyAsync(&v, t->d_checksum, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *checksum = v;
+    }
+    return ATG_OK;
+}
+
+atg_status atg_rejection_build(atg_table* t, void* stream) {
+    if (!t) return fail(ATG_EINVAL, "atg_rejection_build: NULL table");
+    if (!t->weights_set) return fail(ATG_ESTATE, "atg_rejection_build: weights not set");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nb = rejection_blocks(t->n);
+    if (!t->d_A) {
+        t->A_cap = 2 * (uint64_t)t->n + 4096;
+        if (dalloc(&t->d_A, t->A_cap) != hipSuccess) { (void)hipGetLastError(); return fail(ATG_ENOMEM, "rejection A"); }
+        if (dalloc(&t->d_rej_tot, nb) != hipSuccess) { (void)hipGetLastError(); return fail(ATG_ENOMEM, "rejection totals"); }
+        if (dalloc(&t->d_rej_off, nb) != hipSuccess) { (void)hipGetLastError(); return fail(ATG_ENOMEM, "rejection offsets"); }
+    }
+    // W: the canonical sum (RejectionSampling.cu:36 uses the host sum).
+    BuildArgs ba = t->args();
+    HIPCHK(launch_sum(ba, s));
+    RejArgs r{};
+    r.weights = t->d_weights;
+    r.dtype_f32 = t->dtype == ATG_F32;
+    r.n = t->n;
+    r.st = t->d_state;
+    r.nb = nb;
+    r.block_tot = t->d_rej_tot;
+    r.block_off = t->d_rej_off;
+    r.A = t->d_A;
+    r.A_cap = t->A_cap;
+    HIPCHK(launch_rejection_build(r, s));
+    t->rej_built = true;
+    return ATG_OK;
+}
+
+atg_status atg_rejection_size(atg_table* t, uint64_t* a_size) {
+    if (!t || !a_size) return fail(ATG_EINVAL, "atg_rejection_size: NULL argument");
+    DevState st;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
+    *a_size = st.rej_size;
+    return state_status(st);
+}
+
+atg_status atg_get_rejection_array(atg_table* t, int32_t* A, uint64_t cap) {
+    uint64_t sz = 0;
+    atg_status rc = atg_rejection_size(t, &sz);
+    if (rc != ATG_OK) return rc;
+    if (cap < sz) return fail(ATG_EINVAL, "atg_get_rejection_array: capacity %llu < %llu", (unsigned long long)cap, (unsigned long long)sz);
+    HIPCHK(hipMemcpy(A, t->d_A, 4 * sz, hipMemcpyDeviceToHost));
+    return ATG_OK;
+}
+
+atg_status atg_rejection_sample(atg_table* t, uint64_t count, uint64_t seed, int32_t* out, uint32_t* hist,
+                                uint64_t* checksum, void* stream) {
+    if (!t) return fail(ATG_EINVAL, "atg_rejection_sample: NULL table");
+    if (!t->rej_built) return fail(ATG_ESTATE, "atg_rejection_sample: rejection structure not built");
+    hipStream_t s = (hipStream_t)stream;
+    RejArgs r{};
+    r.weights = t->d_weights;
+    r.dtype_f32 = t->dtype == ATG_F32;
+    r.n = t->n;
+    r.st = t->d_state;
+    r.A = t->d_A;
+    r.A_cap = t->A_cap;
+    SampleArgs a{};
+    a.st = t->d_state;
+    a.n = t->n;
+    a.count = count;
+    a.seed = seed;
+    a.out = out;
+    a.hist = hist;
+    a.checksum = checksum ? t->d_checksum : nullptr;
+    if (checksum) HIPCHK(hipMemsetAsync(t->d_checksum, 0, 8, s));
+    if (count > 0) HIPCHK(launch_rejection_sample(r, a, s));
+    if (checksum) {
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpyAsync(&v, t->d_checksum, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *checksum = v;
+    }
+    return ATG_OK;
+}
+
+}  // extern "C"

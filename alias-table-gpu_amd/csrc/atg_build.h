@@ -1,0 +1,52 @@
+// Host-visible description of one build and the launchers of its phases.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "atg_types.h"
+
+namespace atg {
+
+// Everything a build kernel needs, passed by value (all device pointers).
+struct BuildArgs {
+    const void* weights;  // n weights (double or float)
+    int dtype_f32;
+    int64_t n;
+    int64_t G;            // sections
+    int64_t Ls;           // items per section
+    int64_t Ls_max;       // LDS capacity of the partition kernel (items)
+    DevState* st;
+    double* section_sums;     // G
+    double* tree_scratch;     // 2 * tree_scratch_half
+    int64_t tree_scratch_half;
+    uint64_t* slots;          // 6 * G look-back slots: aggC aggL aggH inclC inclL inclH
+    int32_t* Li; double* Lw; double* Lp;  // leftover lights (capacity n + 2)
+    int32_t* Hi; double* Hw; double* Hp;  // leftover heavies (capacity n + 2)
+    Row* rows;                // AoS table or null
+    double* tw; int32_t* ta;  // SoA table or null
+    int32_t psa_plus;
+    int32_t chains;
+    int64_t p_req;            // p after the N/10 clamp (grid bound for split/pack)
+    Split* splits;            // capacity p_req + 1
+    // learned split index
+    int32_t learned_segments;
+    int32_t* knot_j;          // learned_segments + 1 exact split j at the knots
+};
+
+// Section layout of PSA+ (AliasTableSplitGpu.cu:356-357, PsaPlusNormal.cu:27-29).
+inline void section_layout(int64_t n, int64_t param, int64_t* G, int64_t* Ls) {
+    if (param < 1) param = 1;
+    *G = n / param + 1;
+    *Ls = n / *G + 1;
+}
+
+size_t psa_lds_bytes(int64_t Ls_max);
+hipError_t set_partition_lds_limit(int64_t Ls_max);
+hipError_t launch_sum(const BuildArgs& a, hipStream_t s);
+hipError_t launch_partition(const BuildArgs& a, hipStream_t s);
+hipError_t launch_split(const BuildArgs& a, hipStream_t s);
+hipError_t launch_split_learned(const BuildArgs& a, hipStream_t s);
+hipError_t launch_pack(const BuildArgs& a, hipStream_t s);
+
+}  // namespace atg

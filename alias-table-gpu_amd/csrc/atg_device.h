@@ -1,0 +1,132 @@
+// Device-side building blocks shared by the gfx950 kernels: Philox4x32-10,
+// 64-lane wavefront primitives for FP64, the canonical pairwise tree and
+// Kogge-Stone scan, and agent-scope self-validating slots for look-back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "atg_types.h"
+
+#define ATG_WAVE 64
+#define ATG_EPSILON 1e-10  // construction/SamplingAlgorithm.cuh:14
+
+// A published 64-bit slot holds this pattern until its owner writes it.  Sums
+// of validated (finite, non-negative) weights and packed counts (< 2^31 per
+// half) never produce it.
+#define ATG_NOT_READY 0xFFFFFFFFFFFFFFFFull
+
+namespace atg {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Random123 method).  Counter = (idx_lo, idx_hi, purpose, extra),
+// key = (seed_lo, seed_hi).
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint32_t lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    }
+    return c;
+}
+
+__device__ __forceinline__ u32x4 philox_at(uint64_t idx, uint32_t purpose, uint32_t extra, uint64_t seed) {
+    return philox4x32_10(u32x4{(uint32_t)idx, (uint32_t)(idx >> 32), purpose, extra}, (uint32_t)seed,
+                         (uint32_t)(seed >> 32));
+}
+
+__device__ __forceinline__ uint64_t word64(u32x4 r) { return ((uint64_t)r.y << 32) | r.x; }
+__device__ __forceinline__ double coin53(u32x4 r) {
+    const uint64_t v = ((uint64_t)r.w << 32) | r.z;
+    return (double)(v >> 11) * 0x1.0p-53;
+}
+// floor(x * n / 2^64): a uniform row in [0, n) without the 2^-32 bias of a
+// 32-bit multiply-shift.
+__device__ __forceinline__ uint64_t scale64(uint64_t x, uint64_t n) { return __umul64hi(x, n); }
+
+// ---------------------------------------------------------------------------
+// Wavefront primitives (wave64).
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, ATG_WAVE); }
+__device__ __forceinline__ double shfl_up_d(double v, int d) { return __shfl_up(v, d, ATG_WAVE); }
+__device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, ATG_WAVE); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int l = lane_id();
+    return (l == 0) ? 0ull : (~0ull >> (64 - l));
+}
+
+// Pairwise tree over the 64 lanes (butterfly; commutativity makes every lane
+// hold the same bits as t[m] = t[m] + t[m+w] for w = 1, 2, ..., 32).
+__device__ __forceinline__ double wave_tree_sum(double v) {
+#pragma unroll
+    for (int m = 1; m < ATG_WAVE; m <<= 1) v = v + shfl_xor_d(v, m);
+    return v;
+}
+
+// Kogge-Stone inclusive scan, offsets 1, 2, ..., 32.
+__device__ __forceinline__ double wave_ks_inclusive(double v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int off = 1; off < ATG_WAVE; off <<= 1) {
+        const double o = shfl_up_d(v, off);
+        if (l >= off) v = o + v;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum_int(T v) {
+#pragma unroll
+    for (int m = 1; m < ATG_WAVE; m <<= 1) v += __shfl_xor(v, m, ATG_WAVE);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_excl_scan_int(T v) {
+    const int l = lane_id();
+    T inc = v;
+#pragma unroll
+    for (int off = 1; off < ATG_WAVE; off <<= 1) {
+        const T o = __shfl_up(inc, off, ATG_WAVE);
+        if (l >= off) inc += o;
+    }
+    return inc - v;
+}
+
+// ---------------------------------------------------------------------------
+// Agent-scope self-validating slots.  Each slot is written exactly once per
+// build by its owner; readers poll until the value differs from ATG_NOT_READY.
+// Relaxed agent-scope atomics compile to sc1 loads/stores on gfx950, which are
+// coherent across the XCDs without an L2 writeback.
+__device__ __forceinline__ void slot_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t slot_load(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t d2u(double d) { return (uint64_t)__double_as_longlong(d); }
+__device__ __forceinline__ double u2d(uint64_t u) { return __longlong_as_double((long long)u); }
+
+// ---------------------------------------------------------------------------
+// Sampling decision of doSample (sampling/aliasTable/SamplerGpuBasic.cu:148-166):
+// alias iff row.w < W/N and coin * W/N >= row.w.  An alias that is not an item
+// (the pack's heavy sentinel N) keeps the row.
+__device__ __forceinline__ int32_t decide(double rw, int32_t alias, int64_t row, double u, double W_N,
+                                          int64_t n) {
+    return (rw < W_N && u * W_N >= rw && alias >= 0 && alias < n) ? alias : (int32_t)row;
+}
+
+}  // namespace atg

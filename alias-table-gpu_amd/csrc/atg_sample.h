@@ -1,0 +1,57 @@
+// Host-visible argument blocks and launchers of the sampling kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "atg_types.h"
+
+namespace atg {
+
+// Philox stream purposes (word 2 of the counter).
+enum : uint32_t {
+    PURPOSE_BASIC = 0x0000B001u,
+    PURPOSE_SECTIONED = 0x0000B002u,
+    PURPOSE_SECTION_COUNTS = 0x0000B003u,
+    PURPOSE_REJECTION = 0x0000B004u,
+    PURPOSE_BATCH = 0x0000B005u,
+};
+#define REJECTION_MAX_TRIALS 256
+
+struct SampleArgs {
+    const Row* rows;       // AoS table or null
+    const double* tw;      // SoA table or null
+    const int32_t* ta;
+    const DevState* st;    // W/N of the build
+    int64_t n;
+    uint64_t count;
+    uint64_t seed;
+    int32_t* out;          // nullable
+    uint32_t* hist;        // nullable
+    unsigned long long* checksum;  // nullable (device)
+    int64_t SS;            // sectioned: rows per section
+    int64_t nsec;
+};
+
+struct RejArgs {
+    const void* weights;
+    int dtype_f32;
+    int64_t n;
+    DevState* st;          // W (input), rej_size (output)
+    int64_t nb;            // blocks of 4096 items
+    uint64_t* block_tot;   // nb
+    uint64_t* block_off;   // nb
+    uint64_t* incl_out;    // n inclusive offsets (nullable)
+    int32_t* A;
+    uint64_t A_cap;
+    unsigned long long* trials;  // nullable
+};
+
+hipError_t launch_sample_basic(const SampleArgs& a, hipStream_t s);
+hipError_t launch_sample_sectioned(const SampleArgs& a, hipStream_t s);
+hipError_t set_sample_lds_limit(int bytes);
+hipError_t launch_rejection_build(const RejArgs& a, hipStream_t s);
+hipError_t launch_rejection_sample(const RejArgs& a, const SampleArgs& sa, hipStream_t s);
+int64_t rejection_blocks(int64_t n);
+
+}  // namespace atg

@@ -1,0 +1,49 @@
+// Types shared by the host orchestration and the gfx950 kernels.
+#pragma once
+
+#include <stdint.h>
+
+namespace atg {
+
+// Error bits reported through DevState::error.
+enum : uint32_t {
+    ERR_BAD_WEIGHT = 1u,      // NaN, inf or negative weight
+    ERR_BAD_SUM = 2u,         // W not positive / not finite
+    ERR_LOOKBACK_TIMEOUT = 4u,
+    ERR_SECTION_TOO_LONG = 8u,
+};
+
+// Build state kept in device memory so that every phase runs without a host
+// round trip (the reference copies numL/numH/W/totals to the host mid-build,
+// AliasTableSplitGpu.cu:293,330,361-362,476-479).
+struct DevState {
+    double W;
+    double W_N;
+    uint32_t error;
+    uint32_t ticket;          // dynamic section ids of the PSA pass
+    uint32_t degenerate;
+    uint32_t pad0;
+    int64_t numL, numH;       // incl. sentinels
+    int64_t p;                // effective splits
+    int64_t nToSplit;
+    double wToSplit;
+    uint64_t cert_max_dL;     // bit pattern of max light increment (non-negative double)
+    uint64_t cert_min_dH;     // bit pattern of min heavy increment
+    uint64_t split_fallbacks;
+    uint64_t learned_fallbacks;
+    uint64_t rej_size;        // |A| of the rejection structure
+};
+
+// Rows of the AoS table: 16 bytes (TableStorage.cuh:8-11 TableRow plus explicit padding).
+struct alignas(16) Row {
+    double weight;
+    int32_t alias;
+    int32_t pad;
+};
+
+struct Split {
+    int32_t i, j;
+    double spill;
+};
+
+}  // namespace atg

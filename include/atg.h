@@ -1,0 +1,205 @@
+/*
+ * atg: MI355X (gfx950) weighted-sampling engine - C ABI.
+ *
+ * Drop-in boundary for the hot path of ByteHamster/alias-table-gpu: alias-table
+ * construction (W sum, light/heavy partition, PSA+, prefix sums, split search,
+ * pack), sectioned/basic alias sampling and rejection sampling.  Plain pointers
+ * and sizes only; every GPU call takes the caller's hipStream_t (passed as
+ * void*; NULL = the default stream).  All functions return an atg_status; the
+ * message of the last failure on the calling thread is atg_last_error().
+ *
+ * Each declaration names the reference interface it replaces (file:line in the
+ * reference repository).
+ */
+#ifndef ATG_H
+#define ATG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATG_ABI_VERSION 1
+
+typedef enum {
+    ATG_OK = 0,
+    ATG_EINVAL = 1,   /* invalid argument or weights (NaN, inf, negative, W <= 0) */
+    ATG_ENOMEM = 2,   /* device or host allocation failed */
+    ATG_EDEVICE = 3,  /* HIP runtime error */
+    ATG_ESTATE = 4,   /* call out of order (e.g. sample before build) */
+    ATG_ENOTSUP = 5,  /* option not supported */
+    ATG_ETIMEOUT = 6  /* a device-side bounded wait expired (internal error) */
+} atg_status;
+
+typedef enum { ATG_F64 = 0, ATG_F32 = 1 } atg_dtype;     /* weight type; sums are always FP64 */
+typedef enum { ATG_AOS = 0, ATG_SOA = 1 } atg_layout;    /* TableStorage.cuh:6-128 */
+typedef enum { ATG_MEM_HOST = 0, ATG_MEM_DEVICE = 1 } atg_mem;
+
+typedef enum {
+    ATG_SPLIT_BISECT = 0,   /* SplitMethodBasic (SplitMethodBasic.cu:14-50) */
+    ATG_SPLIT_LEARNED = 1   /* piecewise-linear index + exact local search, bisection fallback */
+} atg_split_method;
+
+typedef enum {
+    ATG_SAMPLE_BASIC = 0,      /* SamplerGpuBasic (SamplerGpuBasic.cu:77-133) */
+    ATG_SAMPLE_SECTIONED = 1   /* SamplerGpuSectioned(Shared) (SamplerGpuSectioned.cu:98-166) */
+} atg_sample_kind;
+
+/* Build options.  Replaces the reference's Variant + numThreads
+ * (AliasTableSplitGpu.cuh:60-100) and the compile-time PSA+ constants
+ * (PsaPlusNormal.cu:3, AliasTableSplitGpu.cu:356).  Zero-initialise and call
+ * atg_default_opts() for the reference shapes. */
+typedef struct {
+    int32_t psa_plus;        /* 1: PSA+ section pre-pack (Variant::psaPlus) */
+    int32_t chains;          /* PSA+ greedy chains per section (NUM_LOCAL_SPLITS = 16), 1..64 */
+    int64_t section_size;    /* section parameter; 0 = reference: min(1000, N/10) */
+    int64_t num_splits;      /* p (numThreads); 0 = reference AUTO for the chunked pack: 320 * #CU */
+    int32_t split_method;    /* atg_split_method */
+    int32_t learned_segments;/* knots of the learned split index (default 4096) */
+    int32_t record_phases;   /* 1: hipEvents around sum/partition/prefixsum/split/pack */
+    int32_t reserved;
+} atg_build_opts;
+
+typedef struct {
+    double W;                /* canonical FP64 sum of the weights */
+    double W_N;              /* W / N */
+    int64_t n;
+    int64_t numL, numH;      /* leftover lists incl. sentinels (AliasTableSplitGpu.cuh:126-127) */
+    int64_t R;               /* leftover items entering split/pack: numL + numH - 2 */
+    int64_t p;               /* effective number of splits after the reference's clamps */
+    int64_t num_sections;    /* PSA+/sum sections */
+    int64_t section_len;     /* items per section */
+    int64_t split_fallbacks; /* bisections that never reached FOUND (bounded; reported) */
+    int64_t learned_fallbacks; /* learned-index predictions that fell back to bisection */
+    int32_t degenerate;      /* one leftover class empty: every leftover row is {w, self} */
+    int32_t monotone;        /* split certificate max dL < min dH held */
+    double phase_ms[5];      /* sum, partition(+PSA+ and prefix sums), prefixsum(0: fused), split, pack */
+} atg_stats;
+
+typedef struct atg_table_s atg_table;
+
+/* ---- construction: AliasTableSplitGpu<TableStorage> -------------------- */
+
+/* Reference defaults: PSA+ on, 16 chains, section min(1000, N/10), p AUTO. */
+void atg_default_opts(atg_build_opts* opts);
+
+/* AliasTableSplitGpu(int size, Variant, int numThreads)
+ * (AliasTableSplitGpu.cuh:96-100) + the allocations of preBuild
+ * (AliasTableSplitGpu.cu:206-217).  n in [1, 2^31 - 16). */
+atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout layout,
+                      const atg_build_opts* opts);
+void atg_destroy(atg_table* t);
+
+/* SamplingAlgorithm::weights + the H2D copy of preBuild
+ * (SamplingAlgorithm.cuh:40, AliasTableSplitGpu.cu:206-217).  Device pointers
+ * are used in place (must stay valid until the next set_weights/destroy). */
+atg_status atg_set_weights(atg_table* t, const void* weights, atg_mem mem, void* stream);
+
+/* AliasTableSplitGpu::build (AliasTableSplitGpu.cu:388-468).  Enqueues every
+ * phase on the stream with no host synchronisation.  Invalid weights are
+ * reported by atg_sync / atg_get_stats. */
+atg_status atg_build(atg_table* t, void* stream);
+
+/* Waits for the stream and returns the build's device-side status. */
+atg_status atg_sync(atg_table* t, void* stream);
+
+/* postBuild + verifyTableCorrectness (AliasTableSplitGpu.cu:593-612,
+ * AliasTable.cu:24-65): copies the table to the host and checks the implied
+ * per-item probabilities against the weights (host weights required).
+ * max_rel_err: max |p_table - p_weight| / p_weight. */
+atg_status atg_post_build(atg_table* t, int32_t verify, double* max_rel_err, int32_t* ok);
+
+atg_status atg_get_stats(atg_table* t, atg_stats* st);
+
+/* AliasTable::aliasTable (AliasTable.cuh:23).  Copies row weights and aliases
+ * (SoA form for either layout) to host or device buffers of n entries. */
+atg_status atg_get_table(atg_table* t, double* weights, int32_t* aliases, atg_mem mem, void* stream);
+
+/* Device pointer to the table: AoS rows {double weight; int32 alias; int32 0}
+ * (16 B) or SoA {double weights[n]; int32 aliases[n]}. */
+atg_status atg_table_device_ptr(atg_table* t, void** rows_or_weights, void** aliases_or_null);
+
+/* The splits array (AliasTableSplitGpu.cuh:137, SplitConfig.cuh:4-8): p + 1
+ * entries {i, j, spill}.  Host buffers. */
+atg_status atg_get_splits(atg_table* t, int32_t* i, int32_t* j, double* spill, int64_t capacity);
+
+/* The leftover lists l, h and prefixWeightL/H (AliasTableSplitGpu.cuh:133-136):
+ * numL / numH entries each (host buffers, capacity n + 2). */
+atg_status atg_get_lh(atg_table* t, int32_t* L_item, double* L_w, double* L_pref, int32_t* H_item,
+                      double* H_w, double* H_pref);
+
+/* ---- sampling: Sampler::benchmarkSampling / getSamplingDistribution ---- */
+
+/* SamplerGpuBasic / SamplerGpuSectioned executeBenchmarkSampling
+ * (SamplerGpuBasic.cu:109-133, SamplerGpuSectioned.cu:138-166) and
+ * getSamplingDistribution (SamplerGpu.cu:9-24).  Draws `count` samples from the
+ * built table with Philox4x32-10 (key = seed, counter = output position).
+ * out (device, count int32, nullable) receives the items; hist (device, n
+ * uint32, nullable) is incremented; checksum (host, nullable) gets the sum of
+ * all sampled items.  section_rows: sectioned sampler section (0 = 4096). */
+atg_status atg_sample(atg_table* t, atg_sample_kind kind, uint64_t count, uint64_t seed,
+                      int64_t section_rows, int32_t* out, uint32_t* hist, uint64_t* checksum,
+                      void* stream);
+
+/* ---- rejection sampling: RejectionSampling + SamplerRejection --------- */
+
+/* RejectionSampling::build (RejectionSampling.cu:35-56): builds
+ * A (item i repeated floor(N w_i / W) + 1 times) from the table's weights and
+ * canonical W.  Requires atg_set_weights (a table build is not needed). */
+atg_status atg_rejection_build(atg_table* t, void* stream);
+atg_status atg_rejection_size(atg_table* t, uint64_t* a_size);
+atg_status atg_get_rejection_array(atg_table* t, int32_t* A_host, uint64_t capacity);
+
+/* SamplerRejection::executeBenchmarkSampling (SamplerRejection.cu:63-72). */
+atg_status atg_rejection_sample(atg_table* t, uint64_t count, uint64_t seed, int32_t* out,
+                                uint32_t* hist, uint64_t* checksum, void* stream);
+
+/* ---- input synthesis (SamplingAlgorithm::generateWeights, SamplingAlgorithm.cu:21-71) */
+
+typedef enum {
+    ATG_DIST_SINE = 1, ATG_DIST_BEST_WORST = 2, ATG_DIST_UNIFORM = 3, ATG_DIST_RAMP = 4,
+    ATG_DIST_POWERLAW1 = 5, ATG_DIST_POWERLAW2 = 6, ATG_DIST_ONE_HEAVY = 7, ATG_DIST_ONE_LIGHT = 8,
+    ATG_DIST_POWERLAW1_SHUFFLED = 9, ATG_DIST_POWERLAW05_SHUFFLED = 10,
+    /* workloads of the benchmark configurations */
+    ATG_DIST_UNIT_UNIFORM = 100,  /* w = 1 - u in (0, 1] */
+    ATG_DIST_PARETO = 101,        /* w = (1 - u)^(-1/a), a = param (1.1) */
+    ATG_DIST_EXPONENTIAL = 102,   /* w = -ln(1 - u) */
+    ATG_DIST_HEAVY_MIX = 103,     /* 1% heavy lognormal(10,1) rescaled to carry param (0.9) of the mass, light U(0,1) */
+    ATG_DIST_ZIPF_SHUFFLED = 104  /* w_r = r^-param shuffled */
+} atg_distribution;
+
+/* Host FP64 weights from a Philox4x32-10 stream (key = seed). */
+atg_status atg_generate_weights(int32_t dist, int64_t n, uint64_t seed, double param, double* out);
+
+/* ---- batched build: many independent tables ------------------------------ */
+
+typedef struct atg_batch_s atg_batch;
+
+/* Tables t = 0..ntables-1 own weights[offsets[t], offsets[t+1]).  Each table is
+ * built exactly as atg_build would build it alone with the same options. */
+atg_status atg_batch_create(atg_batch** out, const int64_t* offsets_host, int64_t ntables,
+                            atg_dtype dtype, const atg_build_opts* opts);
+void atg_batch_destroy(atg_batch* b);
+atg_status atg_batch_set_weights(atg_batch* b, const void* weights, atg_mem mem, void* stream);
+atg_status atg_batch_build(atg_batch* b, void* stream);
+atg_status atg_batch_sync(atg_batch* b, void* stream);
+/* Rows (AoS, 16 B) of all tables, concatenated; per-table W. */
+atg_status atg_batch_get_tables(atg_batch* b, double* weights, int32_t* aliases, double* W_per_table);
+/* `per_table` samples from every table; out[t * per_table + k] = item index
+ * local to table t. */
+atg_status atg_batch_sample(atg_batch* b, uint64_t per_table, uint64_t seed, int32_t* out,
+                            void* stream);
+/* Per-table build latency in ns from device timestamps (last build). */
+atg_status atg_batch_latencies(atg_batch* b, double* ns_per_table);
+
+/* ---- misc --------------------------------------------------------------- */
+const char* atg_last_error(void);
+const char* atg_version(void);
+int32_t atg_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATG_H */

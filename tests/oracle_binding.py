@@ -1,0 +1,218 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).  Test-only."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", ORACLE_DIR], check=True, capture_output=True)
+    return C.CDLL(LIB_PATH)
+
+
+L = _load()
+P = C.c_void_p
+
+
+class Opts(C.Structure):
+    _fields_ = [("psa_plus", C.c_int32), ("chains", C.c_int32), ("section_size", C.c_int64),
+                ("num_splits", C.c_int64), ("threads", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("W", C.c_double), ("W_N", C.c_double), ("num_sections", C.c_int64), ("section_len", C.c_int64),
+                ("numL", C.c_int64), ("numH", C.c_int64), ("R", C.c_int64), ("p", C.c_int64),
+                ("wToSplit", C.c_double), ("nToSplit", C.c_int64), ("split_fallbacks", C.c_int64),
+                ("degenerate", C.c_int32), ("monotone", C.c_int32), ("max_dL", C.c_double), ("min_dH", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+L.oracle_philox4x32_10.argtypes = [P, P, P]
+L.oracle_section_layout.argtypes = [C.c_int64, C.c_int64, P, P]
+L.oracle_section_sum.restype = C.c_double
+L.oracle_section_sum.argtypes = [P, C.c_int64]
+L.oracle_tree64_reduce.restype = C.c_double
+L.oracle_tree64_reduce.argtypes = [P, C.c_int64]
+L.oracle_weight_sum.restype = C.c_double
+L.oracle_weight_sum.argtypes = [P, C.c_int64, C.c_int64, P]
+L.oracle_local_scan.restype = C.c_double
+L.oracle_local_scan.argtypes = [P, C.c_int64, P]
+L.oracle_build.restype = C.c_int
+L.oracle_build.argtypes = [P, C.c_int64, C.POINTER(Opts)] + [P] * 11 + [C.POINTER(Stats)]
+L.oracle_split_one.restype = C.c_int
+L.oracle_split_one.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_double, P, C.c_int64, P, P, C.c_int64, P, P, P]
+L.oracle_verify_table.argtypes = [P, C.c_int64, C.c_double, C.c_double, P, P, P, P, P, P]
+L.oracle_sample_basic.argtypes = [P, P, C.c_int64, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, P]
+L.oracle_binomial.restype = C.c_uint64
+L.oracle_binomial.argtypes = [C.c_uint64, C.c_double, C.c_uint64, C.c_uint64]
+L.oracle_section_counts.argtypes = [C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, P]
+L.oracle_sample_sectioned.argtypes = [P, P, C.c_int64, C.c_double, C.c_int64, C.c_uint64, C.c_uint64, P]
+L.oracle_rejection_build.restype = C.c_int
+L.oracle_rejection_build.argtypes = [P, C.c_int64, C.c_double, P, P, P]
+L.oracle_sample_rejection.argtypes = [P, C.c_uint64, P, C.c_int64, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, P, P]
+L.oracle_ref_split_build.argtypes = [P, C.c_int64, C.c_int64, P, P, P]
+L.oracle_ref_sweep_build.argtypes = [P, C.c_int64, P, P, P]
+L.oracle_ref_stack_build.argtypes = [P, C.c_int64, P, P, P]
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    L.oracle_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def section_layout(n, param):
+    g, l = C.c_int64(), C.c_int64()
+    L.oracle_section_layout(int(n), int(param), C.byref(g), C.byref(l))
+    return g.value, l.value
+
+
+def weight_sum(w, section_param):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    G, _ = section_layout(len(w), section_param)
+    sums = np.empty(G)
+    W = L.oracle_weight_sum(ptr(w), len(w), int(section_param), ptr(sums))
+    return W, sums
+
+
+def local_scan(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    ex = np.empty(len(v))
+    agg = L.oracle_local_scan(ptr(v), len(v), ptr(ex))
+    return ex, agg
+
+
+def build(w, psa_plus=True, chains=16, section_size=None, num_splits=81920, threads=8, want_lists=True):
+    """Canonical build; section_size None = the reference's min(1000, N/10)."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    n = len(w)
+    if section_size is None or section_size <= 0:
+        section_size = max(1, min(1000, n // 10))
+    o = Opts(1 if psa_plus else 0, int(chains), int(section_size), int(num_splits), int(threads))
+    st = Stats()
+    rw = np.zeros(n)
+    ra = np.full(n, -1, np.int32)
+    res = {"row_w": rw, "row_alias": ra}
+    lists = [None] * 9
+    if want_lists:
+        cap = n + 2
+        pcap = max(2, min(int(num_splits), n // 10)) + 1
+        lists = [np.zeros(cap, np.int32), np.zeros(cap), np.zeros(cap), np.zeros(cap, np.int32), np.zeros(cap),
+                 np.zeros(cap), np.zeros(pcap, np.int32), np.zeros(pcap, np.int32), np.zeros(pcap)]
+    r = L.oracle_build(ptr(w), n, C.byref(o), ptr(rw), ptr(ra), *[ptr(x) for x in lists], C.byref(st))
+    if r != 0:
+        raise ValueError("oracle_build rejected the input")
+    s = st.as_dict()
+    res["stats"] = s
+    if want_lists:
+        nl, nh, p = s["numL"], s["numH"], s["p"]
+        for k, a, m in zip(["L_item", "L_w", "L_pref", "H_item", "H_w", "H_pref"], lists[:6], [nl] * 3 + [nh] * 3):
+            res[k] = a[:m]
+        if p > 0:
+            res["split_i"], res["split_j"], res["split_spill"] = lists[6][:p + 1], lists[7][:p + 1], lists[8][:p + 1]
+    return res
+
+
+def verify(w, W, W_N, row_w, row_alias):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    rel, ab, oor, ok = C.c_double(), C.c_double(), C.c_int64(), C.c_int32()
+    L.oracle_verify_table(ptr(w), len(w), W, W_N, ptr(np.ascontiguousarray(row_w)),
+                          ptr(np.ascontiguousarray(row_alias, dtype=np.int32)), C.byref(rel), C.byref(ab),
+                          C.byref(oor), C.byref(ok))
+    return {"max_rel": rel.value, "max_abs": ab.value, "alias_out_of_range": oor.value, "reference_ok": bool(ok.value)}
+
+
+def split_one(k, p, nToSplit, wToSplit, Lp, Hp, Hw):
+    i, j, s = C.c_int32(), C.c_int32(), C.c_double()
+    found = L.oracle_split_one(int(k), int(p), int(nToSplit), float(wToSplit), ptr(Lp), len(Lp), ptr(Hp), ptr(Hw),
+                               len(Hp), C.byref(i), C.byref(j), C.byref(s))
+    return bool(found), i.value, j.value, s.value
+
+
+def sample_basic(row_w, row_alias, W_N, seed, count, first=0):
+    out = np.empty(int(count), np.int32)
+    L.oracle_sample_basic(ptr(row_w), ptr(row_alias), len(row_w), W_N, int(seed), int(first), int(count), ptr(out))
+    return out
+
+
+def binomial(n, q, seed, node):
+    return L.oracle_binomial(int(n), float(q), int(seed), int(node))
+
+
+def section_counts(n, section_rows, total, seed):
+    nsec = (n + section_rows - 1) // section_rows
+    c = np.empty(nsec, np.uint64)
+    L.oracle_section_counts(int(n), int(section_rows), int(total), int(seed), ptr(c))
+    return c
+
+
+def sample_sectioned(row_w, row_alias, W_N, section_rows, total, seed):
+    out = np.empty(int(total), np.int32)
+    L.oracle_sample_sectioned(ptr(row_w), ptr(row_alias), len(row_w), W_N, int(section_rows), int(total), int(seed),
+                              ptr(out))
+    return out
+
+
+def rejection_build(w, W):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    incl = np.empty(len(w), np.uint64)
+    sz = C.c_uint64()
+    L.oracle_rejection_build(ptr(w), len(w), W, ptr(incl), None, C.byref(sz))
+    A = np.empty(sz.value, np.int32)
+    L.oracle_rejection_build(ptr(w), len(w), W, ptr(incl), ptr(A), C.byref(sz))
+    return incl, A
+
+
+def sample_rejection(A, w, W, seed, count, first=0):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.empty(int(count), np.int32)
+    tr = C.c_uint64()
+    L.oracle_sample_rejection(ptr(A), len(A), ptr(w), len(w), W, int(seed), int(first), int(count), ptr(out),
+                              C.byref(tr))
+    return out, tr.value
+
+
+def ref_split_build(w, p):
+    n = len(w)
+    wt = np.zeros(n + 2)
+    wt[:n] = w
+    rw = np.zeros(n)
+    ra = np.full(n, -1, np.int32)
+    W = C.c_double()
+    L.oracle_ref_split_build(ptr(wt), n, int(p), C.byref(W), ptr(rw), ptr(ra))
+    return W.value, rw, ra
+
+
+def ref_sweep_build(w):
+    n = len(w)
+    wt = np.zeros(n + 2)
+    wt[:n] = w
+    rw = np.zeros(n)
+    ra = np.full(n, -1, np.int32)
+    W = C.c_double()
+    L.oracle_ref_sweep_build(ptr(wt), n, C.byref(W), ptr(rw), ptr(ra))
+    return W.value, rw, ra
+
+
+def ref_stack_build(w):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    n = len(w)
+    rw = np.zeros(n)
+    ra = np.full(n, -1, np.int32)
+    W = C.c_double()
+    L.oracle_ref_stack_build(ptr(w), n, C.byref(W), ptr(rw), ptr(ra))
+    return W.value, rw, ra

@@ -1,0 +1,143 @@
+"""GPU build parity: every stage of the gfx950 build against the CPU oracle,
+bit for bit (W, leftover lists, prefix sums, splits, table rows)."""
+import numpy as np
+import pytest
+
+import atg
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def gpu_build(w, dtype="f64", **kw):
+    t = atg.Table(len(w), dtype=dtype, **kw)
+    t.set_weights(w)
+    t.build()
+    t.sync()
+    return t
+
+
+def assert_same_build(t, ref, n, check_lists=True):
+    st = t.stats()
+    rs = ref["stats"]
+    assert bits([st["W"]])[0] == bits([rs["W"]])[0], (st["W"], rs["W"])
+    assert bits([st["W_N"]])[0] == bits([rs["W_N"]])[0]
+    assert (st["numL"], st["numH"], st["p"], st["degenerate"]) == (rs["numL"], rs["numH"], rs["p"], rs["degenerate"])
+    assert st["monotone"] == rs["monotone"]
+    if check_lists:
+        lh = t.lh()
+        for k in ("L_item", "H_item"):
+            np.testing.assert_array_equal(lh[k], ref[k], err_msg=k)
+        for k in ("L_w", "L_pref", "H_w", "H_pref"):
+            mism = np.nonzero(bits(lh[k]) != bits(ref[k]))[0]
+            assert mism.size == 0, f"{k}: {mism.size} mismatches, first {mism[:5]} gpu={lh[k][mism[:3]]} ref={ref[k][mism[:3]]}"
+        if rs["p"] > 0:
+            i, j, s = t.splits()
+            np.testing.assert_array_equal(i, ref["split_i"])
+            np.testing.assert_array_equal(j, ref["split_j"])
+            assert np.array_equal(bits(s), bits(ref["split_spill"]))
+    tw, ta = t.table()
+    bad = np.nonzero((bits(tw) != bits(ref["row_w"])) | (ta != ref["row_alias"]))[0]
+    assert bad.size == 0, f"{bad.size} table rows differ, first {bad[:5]}: gpu {tw[bad[:3]]} {ta[bad[:3]]} ref {ref['row_w'][bad[:3]]} {ref['row_alias'][bad[:3]]}"
+
+
+CASES = [
+    # n, dist, psa_plus, chains, section, p
+    (1, "unit_uniform", True, 16, 0, 0),
+    (2, "unit_uniform", True, 16, 0, 0),
+    (3, "pareto", False, 16, 0, 0),
+    (10, "unit_uniform", True, 16, 0, 0),
+    (11, "pareto", True, 16, 0, 0),
+    (100, "pareto", True, 16, 0, 0),
+    (1000, "unit_uniform", False, 16, 0, 0),
+    (1000, "pareto", True, 16, 0, 0),
+    (10007, "pareto", True, 16, 0, 0),
+    (10007, "exponential", False, 16, 0, 1400),
+    (65536, "pareto", True, 64, 4096, 0),
+    (100003, "sine", True, 16, 0, 1400),
+    (100003, "powerlaw1_shuffled", True, 8, 2000, 0),
+    (505711, "one_light", True, 16, 0, 16576),
+    (1 << 20, "unit_uniform", True, 16, 0, 0),
+    (1 << 20, "pareto", True, 16, 0, 0),
+    (1 << 20, "pareto", False, 16, 0, 1400),
+    (1 << 20, "heavy_mix", True, 32, 4096, 0),
+]
+
+
+@pytest.mark.parametrize("n,dist,psa,chains,section,p", CASES)
+def test_build_matches_oracle(gpu, oracle, n, dist, psa, chains, section, p):
+    w = atg.generate_weights(dist, n, seed=42, param=1.1 if dist == "pareto" else 0.0)
+    # p = 0 asks for the reference's AUTO value: 320 splits per compute unit.
+    p_req = p if p > 0 else 320 * gpu.cuda.get_device_properties(0).multi_processor_count
+    t = gpu_build(w, psa_plus=psa, chains=chains, section_size=section, num_splits=p_req)
+    ref = oracle.build(w, psa_plus=psa, chains=chains, section_size=section, num_splits=p_req)
+    assert_same_build(t, ref, n)
+
+
+@pytest.mark.parametrize("layout", ["aos", "soa"])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_layout_and_dtype(gpu, oracle, layout, dtype):
+    n = 200003
+    w = atg.generate_weights("pareto", n, seed=7, param=1.1)
+    if dtype == "f32":
+        w = w.astype(np.float32)
+    t = gpu_build(w, dtype=dtype, layout=layout)
+    ref = oracle.build(w.astype(np.float64), num_splits=81920)
+    assert_same_build(t, ref, n)
+
+
+def test_device_weights_and_rebuild(gpu, oracle):
+    torch = gpu
+    n = 300000
+    w = atg.generate_weights("unit_uniform", n, seed=3)
+    dw = torch.from_numpy(w).cuda()
+    t = atg.Table(n)
+    t.set_weights(dw)
+    for _ in range(3):  # repeated builds on the same handle are identical
+        t.build()
+    t.sync()
+    assert_same_build(t, oracle.build(w, num_splits=81920), n)
+
+
+@pytest.mark.parametrize("case", ["all_equal", "zeros_and_ones", "one_heavy", "best_worst", "integers"])
+def test_edge_inputs(gpu, oracle, case):
+    n = 50000
+    if case == "all_equal":
+        w = np.full(n, 3.25)
+    elif case == "zeros_and_ones":
+        w = np.zeros(n)
+        w[::3] = 1.0
+    elif case == "one_heavy":
+        w = atg.generate_weights("one_heavy", n)
+    elif case == "best_worst":
+        w = atg.generate_weights("best_worst", n)
+    else:
+        w = (np.arange(n) % 16).astype(np.float64)
+    t = gpu_build(w)
+    ref = oracle.build(w, num_splits=81920)
+    assert_same_build(t, ref, n)
+    v = oracle.verify(w, ref["stats"]["W"], ref["stats"]["W_N"], ref["row_w"], ref["row_alias"])
+    assert v["reference_ok"]
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -1.0])
+def test_invalid_weights_rejected(gpu, bad):
+    w = np.ones(1000)
+    w[17] = bad
+    t = atg.Table(1000)
+    t.set_weights(w)
+    t.build()
+    with pytest.raises(atg.AtgError) as e:
+        t.sync()
+    assert e.value.status == 1
+
+
+def test_all_zero_weights_rejected(gpu):
+    t = atg.Table(100)
+    t.set_weights(np.zeros(100))
+    t.build()
+    with pytest.raises(atg.AtgError):
+        t.sync()

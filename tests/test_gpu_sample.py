@@ -1,0 +1,95 @@
+"""GPU sampling parity: basic, sectioned and rejection samplers produce the
+oracle's items on the same Philox stream; histograms pass chi-squared."""
+import numpy as np
+import pytest
+from scipy import stats as sstats
+
+import atg
+
+pytestmark = pytest.mark.gpu
+
+
+def build_pair(oracle, n, dist="pareto", seed=42, **kw):
+    w = atg.generate_weights(dist, n, seed=seed, param=1.1 if dist == "pareto" else 0.0)
+    t = atg.Table(n, **kw)
+    t.set_weights(w)
+    t.build()
+    t.sync()
+    ref = oracle.build(w, num_splits=81920, want_lists=False)
+    tw, ta = t.table()
+    assert np.array_equal(tw.view(np.uint64), ref["row_w"].view(np.uint64))
+    assert np.array_equal(ta, ref["row_alias"])
+    return w, t, ref
+
+
+def chi2_pvalue(counts, w):
+    expected = w / w.sum() * counts.sum()
+    keep = expected >= 5
+    obs = np.append(counts[keep], counts[~keep].sum())
+    exp = np.append(expected[keep], expected[~keep].sum())
+    if exp[-1] == 0:
+        obs, exp = obs[:-1], exp[:-1]
+    return sstats.chisquare(obs, exp * obs.sum() / exp.sum()).pvalue
+
+
+@pytest.mark.parametrize("n,count", [(1000, 100000), (100003, 2000000), (1 << 20, 3000000)])
+def test_basic_matches_oracle(gpu, oracle, n, count):
+    torch = gpu
+    w, t, ref = build_pair(oracle, n)
+    out = torch.empty(count, dtype=torch.int32, device="cuda")
+    cs = t.sample(count, seed=123, kind="basic", out=out, checksum=True)
+    got = out.cpu().numpy()
+    exp = oracle.sample_basic(ref["row_w"], ref["row_alias"], ref["stats"]["W_N"], 123, count)
+    np.testing.assert_array_equal(got, exp)
+    assert cs == int(exp.astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("n,count,rows", [(1000, 100000, 64), (100003, 2000000, 4096), (1 << 20, 3000000, 4096),
+                                          (777777, 1000, 1024), (5000, 7, 4096)])
+def test_sectioned_matches_oracle(gpu, oracle, n, count, rows):
+    torch = gpu
+    w, t, ref = build_pair(oracle, n)
+    out = torch.empty(max(count, 1), dtype=torch.int32, device="cuda")
+    t.sample(count, seed=99, kind="sectioned", section_rows=rows, out=out)
+    got = out.cpu().numpy()[:count]
+    exp = oracle.sample_sectioned(ref["row_w"], ref["row_alias"], ref["stats"]["W_N"], rows, count, 99)
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_histograms_chi_squared(gpu, oracle):
+    torch = gpu
+    n = 10000
+    w = atg.generate_weights("sine", n)
+    t = atg.Table(n)
+    t.set_weights(w)
+    t.build()
+    t.sync()
+    for kind in ("basic", "sectioned"):
+        hist = torch.zeros(n, dtype=torch.int32, device="cuda")
+        t.sample(20_000_000, seed=5, kind=kind, section_rows=2500, hist=hist)
+        h = hist.cpu().numpy().astype(np.float64)
+        assert h.sum() == 20_000_000
+        assert chi2_pvalue(h, w) > 1e-4, kind
+    t.rejection_build()
+    hist = torch.zeros(n, dtype=torch.int32, device="cuda")
+    t.rejection_sample(20_000_000, seed=6, hist=hist)
+    h = hist.cpu().numpy().astype(np.float64)
+    assert chi2_pvalue(h, w) > 1e-4
+
+
+@pytest.mark.parametrize("n,dist", [(1000, "sine"), (100003, "pareto"), (1 << 20, "exponential")])
+def test_rejection_matches_oracle(gpu, oracle, n, dist):
+    torch = gpu
+    w = atg.generate_weights(dist, n, seed=11, param=1.1 if dist == "pareto" else 0.0)
+    t = atg.Table(n)
+    t.set_weights(w)
+    t.rejection_build()
+    W = oracle.weight_sum(w, max(1, min(1000, n // 10)))[0]
+    incl, A = oracle.rejection_build(w, W)
+    assert t.rejection_size() == len(A)
+    np.testing.assert_array_equal(t.rejection_array(), A)
+    count = 1_000_000
+    out = torch.empty(count, dtype=torch.int32, device="cuda")
+    t.rejection_sample(count, seed=8, out=out)
+    exp, _ = oracle.sample_rejection(A, w, W, 8, count)
+    np.testing.assert_array_equal(out.cpu().numpy(), exp)
