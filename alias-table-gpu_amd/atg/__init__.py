@@ -95,6 +95,12 @@ class Table:
     def build(self, stream=None):
         check(lib().atg_build(self.h, _stream(stream)))
 
+    PHASES = ("sum", "partition", "split", "pack")
+
+    def build_phase(self, phase, stream=None):
+        """One phase of build(): 'sum', 'partition' (PSA+ and prefix sums), 'split' or 'pack'."""
+        check(lib().atg_build_phase(self.h, self.PHASES.index(phase), _stream(stream)))
+
     def sync(self, stream=None):
         check(lib().atg_sync(self.h, _stream(stream)))
 

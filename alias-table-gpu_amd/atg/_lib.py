@@ -51,6 +51,7 @@ SIGNATURES = [
     ("atg_destroy", None, [_P]),
     ("atg_set_weights", _I32, [_P, _P, _I32, _P]),
     ("atg_build", _I32, [_P, _P]),
+    ("atg_build_phase", _I32, [_P, _I32, _P]),
     ("atg_sync", _I32, [_P, _P]),
     ("atg_post_build", _I32, [_P, _I32, C.POINTER(_D), C.POINTER(_I32)]),
     ("atg_get_stats", _I32, [_P, C.POINTER(Stats)]),

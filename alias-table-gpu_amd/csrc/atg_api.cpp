@@ -256,22 +256,41 @@ atg_status atg_set_weights(atg_table* t, const void* w, atg_mem mem, void* strea
     return ATG_OK;
 }
 
+atg_status atg_build_phase(atg_table* t, int32_t phase, void* stream) {
+    if (!t) return fail(ATG_EINVAL, "atg_build_phase: NULL table");
+    if (!t->weights_set) return fail(ATG_ESTATE, "atg_build_phase: weights not set");
+    hipStream_t s = (hipStream_t)stream;
+    const BuildArgs a = t->args();
+    switch (phase) {
+        case ATG_PHASE_SUM: HIPCHK(launch_sum(a, s)); break;
+        case ATG_PHASE_PARTITION: HIPCHK(launch_partition(a, s)); break;
+        case ATG_PHASE_SPLIT:
+            if (t->opts.split_method == ATG_SPLIT_LEARNED) HIPCHK(launch_split_learned(a, s));
+            else HIPCHK(launch_split(a, s));
+            break;
+        case ATG_PHASE_PACK:
+            HIPCHK(launch_pack(a, s));
+            t->built = true;
+            break;
+        default: return fail(ATG_EINVAL, "atg_build_phase: unknown phase %d", phase);
+    }
+    return ATG_OK;
+}
+
 atg_status atg_build(atg_table* t, void* stream) {
     if (!t) return fail(ATG_EINVAL, "atg_build: NULL table");
     if (!t->weights_set) return fail(ATG_ESTATE, "atg_build: weights not set");
     hipStream_t s = (hipStream_t)stream;
-    const BuildArgs a = t->args();
-    if (t->events) HIPCHK(hipEventRecord(t->ev[0], s));
-    HIPCHK(launch_sum(a, s));
-    if (t->events) HIPCHK(hipEventRecord(t->ev[1], s));
-    HIPCHK(launch_partition(a, s));
-    if (t->events) HIPCHK(hipEventRecord(t->ev[2], s));
-    if (t->events) HIPCHK(hipEventRecord(t->ev[3], s));
-    HIPCHK(launch_split(a, s));
-    if (t->events) HIPCHK(hipEventRecord(t->ev[4], s));
-    HIPCHK(launch_pack(a, s));
+    // Events follow TableSplitTimer's phases (utils/Timer.cu:62-77); the
+    // prefix sums are fused into the partition, so that phase reads 0.
+    static const int before[4] = {0, 1, 3, 4};  // event recorded before each phase
+    for (int ph = 0; ph < 4; ++ph) {
+        if (t->events && ph == 2) HIPCHK(hipEventRecord(t->ev[2], s));
+        if (t->events) HIPCHK(hipEventRecord(t->ev[before[ph]], s));
+        atg_status rc = atg_build_phase(t, ph, stream);
+        if (rc != ATG_OK) return rc;
+    }
     if (t->events) HIPCHK(hipEventRecord(t->ev[5], s));
-    t->built = true;
     return ATG_OK;
 }
 

@@ -45,6 +45,7 @@ size_t psa_lds_bytes(int64_t Ls_max);
 hipError_t set_partition_lds_limit(int64_t Ls_max);
 hipError_t launch_sum(const BuildArgs& a, hipStream_t s);
 hipError_t launch_partition(const BuildArgs& a, hipStream_t s);
+hipError_t launch_degenerate(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split_learned(const BuildArgs& a, hipStream_t s);
 hipError_t launch_pack(const BuildArgs& a, hipStream_t s);

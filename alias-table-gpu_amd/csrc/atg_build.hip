@@ -556,12 +556,18 @@ hipError_t launch_partition(const BuildArgs& a, hipStream_t s) {
         if (a.rows) hipLaunchKernelGGL((psa_sections<double, true>), g, b, lds, s, a);
         else hipLaunchKernelGGL((psa_sections<double, false>), g, b, lds, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_degenerate(const BuildArgs& a, hipStream_t s) {
     if (a.rows) hipLaunchKernelGGL(degenerate_fill<true>, dim3(1024), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(degenerate_fill<false>, dim3(1024), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_split(const BuildArgs& a, hipStream_t s) {
+    hipError_t e = launch_degenerate(a, s);
+    if (e != hipSuccess) return e;
     const int64_t kmax = a.p_req;  // effective p <= p_req
     if (kmax >= 2) hipLaunchKernelGGL(split_bisect, dim3(blocks_for(kmax - 1, 256)), dim3(256), 0, s, a);
     return hipGetLastError();

@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the alias-table hot path on one MI355X.
+
+Workload (the headline configuration): N = 1e8 FP64 weights i.i.d.
+Pareto(shape 1.1, scale 1) from Philox seed 42, plain PSA+ build (W sum,
+light/heavy partition + PSA+ with fused leftover prefix sums, split, pack),
+then 1e9 sectioned samples (2^12-row sections) and the rejection-sampling
+variant.  One step = one full build over the resident weights.
+
+Prints one JSON line.  `value` is build throughput (Gitems/s, weights already in
+HBM); `roofline` is the dominant build kernel's algorithmic bytes over its mean
+duration measured with HIP events on the build stream; `cpu_baseline` is the CPU
+oracle (the same canonical build) on the host's cores over a bounded sample.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "alias-table-gpu_amd"))
+
+METRIC = "build Gitems/s + Gsamples/s at N=1e8 (%HBM peak); learned/batched vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def phase_bytes(n, R, p, row_bytes=16, w_bytes=8):
+    """Algorithmic HBM bytes of one launch of each build phase.
+
+    Leftover lists are SoA: item (4 B) + weight (8 B) + prefix (8 B) written,
+    item + weight (12 B) read by the pack."""
+    return {
+        "sum": n * w_bytes,
+        "partition": n * w_bytes + (n - R) * row_bytes + R * 20,
+        "split": p * 2 * 16,  # the final probe pair per split; the search itself is latency-bound
+        "pack": R * 12 + R * row_bytes,
+    }
+
+
+def cpu_baseline(n_cpu, args):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import atg
+    import oracle_binding as oracle
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(cores, 64))
+    w = atg.generate_weights("pareto", n_cpu, seed=42, param=1.1)
+    times = []
+    for rep in range(3):
+        t0 = time.perf_counter()
+        oracle.build(w, psa_plus=True, chains=args.chains, section_size=args.section_size,
+                     num_splits=args.num_splits or 81920, threads=threads, want_lists=False)
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    # faithful single-thread restatement of the reference CPU PSA (AliasTableSplit::build)
+    t0 = time.perf_counter()
+    oracle.ref_split_build(w, 1400)
+    t_ref = time.perf_counter() - t0
+    return {"value": round(n_cpu / t / 1e9, 4), "unit": "Gitems/s", "cores": threads, "kind": "port",
+            "sample": f"canonical PSA+ build of N={n_cpu:.0e} Pareto(1.1) FP64 weights (same generator, seed 42), "
+                      f"OpenMP over {threads} threads, median of 3",
+            "single_thread_reference_psa_gitems_s": round(n_cpu / t_ref / 1e9, 4)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=100_000_000)
+    ap.add_argument("--samples", type=int, default=1_000_000_000)
+    ap.add_argument("--section-size", type=int, default=0, help="PSA+ section (0 = reference min(1000, N/10))")
+    ap.add_argument("--chains", type=int, default=16)
+    ap.add_argument("--num-splits", type=int, default=0, help="p (0 = reference AUTO, 320 per CU)")
+    ap.add_argument("--sample-section", type=int, default=4096)
+    ap.add_argument("--cpu-n", type=int, default=10_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sampling", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard into "
+              f"independent per-GPU tables, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import atg
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n = args.n
+    log(f"[bench] generating N={n} Pareto(1.1) weights")
+    w = atg.generate_weights("pareto", n, seed=42, param=1.1)
+    stream = torch.cuda.current_stream()
+    t = atg.Table(n, psa_plus=True, chains=args.chains, section_size=args.section_size,
+                  num_splits=args.num_splits)
+    t.set_weights(w, stream)
+    for _ in range(args.warmup):
+        t.build(stream)
+    torch.cuda.synchronize()
+    t.sync()
+    log(f"[bench] warmup done; timing {args.steps} builds")
+
+    K = args.steps
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(K)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(K):
+        e = evs[k]
+        e[0].record(stream)
+        for i, ph in enumerate(atg.Table.PHASES):
+            t.build_phase(ph, stream)
+            e[i + 1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    t.sync()
+    st = t.stats()
+    ms_per_step = (t1 - t0) * 1e3 / K
+    phase_ms = {ph: statistics.mean(evs[k][i].elapsed_time(evs[k][i + 1]) for k in range(K))
+                for i, ph in enumerate(atg.Table.PHASES)}
+    build_ms_events = statistics.mean(evs[k][0].elapsed_time(evs[k][4]) for k in range(K))
+    R, p = st["R"], st["p"]
+    pb = phase_bytes(n, R, p)
+    dom = max(("sum", "partition", "pack"), key=lambda ph: phase_ms[ph])
+    achieved = pb[dom] / (phase_ms[dom] * 1e-3) / 1e9
+    b_build = n * (2 * 8 + 16) + R * (2 * 12 + 8)
+    log(f"[bench] build {ms_per_step:.3f} ms/step (events {build_ms_events:.3f}); phases {phase_ms}; R/N={R / n:.3f} p={p}")
+
+    result = {
+        "metric": METRIC,
+        "value": round(n * K / (t1 - t0) / 1e9, 3),
+        "unit": "Gitems/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain PSA+ alias-table build, N=1e8 FP64 Pareto(1.1) weights (seed 42), "
+                               "then 1e9 sectioned samples (2^12-row sections) + rejection variant",
+                   "n": n, "section_size": st["section_len"], "chains": args.chains, "p": p, "layout": "aos",
+                   "psa_plus": True},
+        "roofline": {"bound": "hbm", "kernel": {"partition": "psa_sections", "sum": "sum_sections+finalize_sum",
+                                                "pack": "pack_basic"}[dom],
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": pb[dom], "mean_ms": round(phase_ms[dom], 4)},
+        "build": {"phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
+                  "R": R, "R_over_N": round(R / n, 4), "P_w": 2, "B_build": b_build,
+                  "build_hbm_frac": round(b_build / (build_ms_events * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "floor_fraction": round(n * 32 / (build_ms_events * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "W": st["W"], "monotone": st["monotone"], "split_fallbacks": st["split_fallbacks"]},
+    }
+
+    if not args.no_sampling:
+        S = args.samples
+        out = torch.empty(S, dtype=torch.int32, device=dev)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        t.sample(S, seed=1, kind="sectioned", section_rows=args.sample_section, out=out, stream=stream)
+        ts = []
+        for rep in range(3):
+            ev[0].record(stream)
+            t.sample(S, seed=100 + rep, kind="sectioned", section_rows=args.sample_section, out=out, stream=stream)
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]))
+        ms = statistics.median(ts)
+        sb = 4 * S + 16 * n
+        result["sampling"] = {"sectioned_gsamples_s": round(S / (ms * 1e-3) / 1e9, 3), "sectioned_ms": round(ms, 3),
+                              "samples": S, "section_rows": args.sample_section,
+                              "sectioned_hbm_frac": round(sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"[bench] sectioned sampling {S} in {ms:.3f} ms")
+        ev[0].record(stream)
+        t.rejection_build(stream)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        rb_ms = ev[0].elapsed_time(ev[1])
+        A = t.rejection_size()
+        t.rejection_sample(S, seed=3, out=out, stream=stream)
+        ts = []
+        for rep in range(3):
+            ev[0].record(stream)
+            t.rejection_sample(S, seed=10 + rep, out=out, stream=stream)
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]))
+        rms = statistics.median(ts)
+        rbytes = 4 * S + (A / n) * S * 12
+        result["sampling"].update({"rejection_build_ms": round(rb_ms, 3), "rejection_A_over_N": round(A / n, 4),
+                                   "rejection_gsamples_s": round(S / (rms * 1e-3) / 1e9, 3),
+                                   "rejection_hbm_frac": round(rbytes / (rms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
+        log(f"[bench] rejection build {rb_ms:.3f} ms, sampling {rms:.3f} ms")
+        del out
+
+    if not args.no_cpu_baseline:
+        log(f"[bench] CPU baseline on N={args.cpu_n}")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_n, args)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -102,6 +102,13 @@ atg_status atg_set_weights(atg_table* t, const void* weights, atg_mem mem, void*
  * reported by atg_sync / atg_get_stats. */
 atg_status atg_build(atg_table* t, void* stream);
 
+/* One phase of build, in order SUM, PARTITION (PSA+ with the fused leftover
+ * prefix sums), SPLIT, PACK: the reference exposes performSplit/performPack
+ * the same way (AliasTableSplitGpu.cuh:107-115) and times the phases with
+ * TableSplitTimer (utils/Timer.cuh:17-45).  atg_build == all four in order. */
+typedef enum { ATG_PHASE_SUM = 0, ATG_PHASE_PARTITION = 1, ATG_PHASE_SPLIT = 2, ATG_PHASE_PACK = 3 } atg_phase;
+atg_status atg_build_phase(atg_table* t, int32_t phase, void* stream);
+
 /* Waits for the stream and returns the build's device-side status. */
 atg_status atg_sync(atg_table* t, void* stream);
 
