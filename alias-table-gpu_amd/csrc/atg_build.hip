@@ -2,36 +2,37 @@
 // AliasTableSplitGpu::build, AliasTableSplitGpu.cu:388-468).
 //
 // Phases (one stream, no host round trips):
-//   1. sum_sections   : per-section FP64 sums (one wave per section) + weight
-//                       validation + reset of the look-back slots.
-//   2. finalize_sum   : one workgroup reduces the section sums with the
-//                       canonical 64-ary pairwise tree -> W, W/N.
-//   3. psa_sections   : one wave per section, taken in ticket order: stable
-//                       light/heavy partition in LDS, PSA+ greedy chains that
-//                       write finished rows, leftover compaction, the canonical
-//                       local scan of the leftover weights and a decoupled
-//                       look-back that folds predecessors left to right (bit
-//                       identical to the chained prefix), leftover lists +
-//                       prefix sums, the split certificate and the sentinels.
-//   4. degenerate_fill: only when one leftover class is empty.
-//   5. split          : bisection per k (SplitMethodBasic.cu:14-50).
-//   6. pack           : one chain per k (PackMethodBasic.cuh:59-126).
+//   1. sum_sections : per-section FP64 sums (64 sections per block), weight
+//                     validation, the first level of the canonical 64-ary tree;
+//                     the last block to finish completes the tree -> W, W/N.
+//   2. psa_tiles    : a 4-wave workgroup per tile of T sections, tiles taken
+//                     in ticket order; per section: all loads issued at once,
+//                     stable light/heavy partition into LDS, PSA+ greedy
+//                     chains (spread over the 4 waves) writing finished rows,
+//                     leftover maps, the canonical local scan of the leftover
+//                     weights, a decoupled look-back over tiles that folds the
+//                     predecessors' section aggregates left to right (bit
+//                     identical to the chained prefix P_{s+1} = P_s + agg_s),
+//                     leftover lists + prefix sums, the split certificate and
+//                     the sentinels.
+//   3. degenerate_fill (only when one leftover class is empty), split, pack.
 #include "atg_build.h"
 #include "atg_device.h"
 
 namespace atg {
 
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // ---------------------------------------------------------------------------
 // 1. Section sums.  Lane l of the wave that owns a section adds items
-//    l, l+64, ... in order; the lane sums meet in the pairwise tree.
-//    A NaN / infinite / negative weight turns the section sum into NaN.
+//    l, l+64, ... in order; the lane sums meet in the pairwise tree.  A NaN,
+//    infinite or negative weight turns the section sum into NaN.
 template <typename T>
-__global__ __launch_bounds__(256) void sum_sections(BuildArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (s >= a.G) return;
-    const T* w = (const T*)a.weights;
-    const int64_t start = min(a.n, s * a.Ls), end = min(a.n, (s + 1) * a.Ls);
+__device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t end, int lane) {
     double acc = 0.0;
     bool bad = false;
     int64_t k = start + lane;
@@ -52,35 +53,63 @@ __global__ __launch_bounds__(256) void sum_sections(BuildArgs a) {
     }
     double sum = wave_tree_sum(acc);
     if (__ballot(bad) != 0ull) sum = __builtin_nan("");
-    if (lane == 0) a.section_sums[s] = sum;
-    if (lane < 6) a.slots[(int64_t)lane * a.G + s] = ATG_NOT_READY;
+    return sum;
 }
 
-// ---------------------------------------------------------------------------
-// 2. W = recursive 64-ary pairwise tree over the section sums.
-__global__ __launch_bounds__(1024) void finalize_sum(BuildArgs a) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    const double* src = a.section_sums;
+// Block b owns sections [64 b, 64 b + 64) (16 waves x 4 sections) and writes
+// the tree of their sums (level 1 of the 64-ary tree).  The last block to
+// finish reduces the level-1 values and publishes W.
+template <typename T>
+__global__ __launch_bounds__(1024) void sum_sections(BuildArgs a) {
+    __shared__ double sh[64];
+    __shared__ uint32_t sh_last;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const T* w = (const T*)a.weights;
+    for (int q = 0; q < 4; ++q) {
+        const int idx = wave * 4 + q;
+        const int64_t s = (int64_t)blockIdx.x * 64 + idx;
+        double sum = 0.0;
+        if (s < a.G) sum = section_sum<T>(w, min(a.n, s * a.Ls), min(a.n, (s + 1) * a.Ls), lane);
+        if (lane == 0) sh[idx] = sum;
+    }
+    // Reset the look-back slots of the partition pass.
+    const int64_t nslots = 3 * a.G + 3 * a.Gt;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += (int64_t)gridDim.x * blockDim.x)
+        a.slots[i] = ATG_NOT_READY;
+    __syncthreads();
+    if (wave == 0) {
+        const double r = wave_tree_sum(sh[lane]);
+        if (lane == 0) {
+            a.tree_scratch[blockIdx.x] = r;
+            // Release: the level-1 value is visible before the counter moves.
+            const uint32_t done = __hip_atomic_fetch_add(&a.st->sum_done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            sh_last = (done == gridDim.x - 1) ? 1u : 0u;
+        }
+    }
+    __syncthreads();
+    if (!sh_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // Last block: the remaining levels of the tree over tree_scratch.
     double* bufs[2] = {a.tree_scratch, a.tree_scratch + a.tree_scratch_half};
-    int flip = 0;
-    int64_t count = a.G;
+    int cur = 0;
+    int64_t count = gridDim.x;
+    const int nwaves = blockDim.x >> 6;
     while (count > 1) {
         const int64_t groups = (count + 63) / 64;
-        double* dst = bufs[flip];
+        const double* src = bufs[cur];
+        double* dst = bufs[cur ^ 1];
         for (int64_t g = wave; g < groups; g += nwaves) {
-            const int64_t idx = g * 64 + lane;
-            const double v = (idx < count) ? src[idx] : 0.0;
-            const double r = wave_tree_sum(v);
+            const int64_t i = g * 64 + lane;
+            const double r = wave_tree_sum((i < count) ? src[i] : 0.0);
             if (lane == 0) dst[g] = r;
         }
         __threadfence_block();
         __syncthreads();
-        src = dst;
-        flip ^= 1;
+        cur ^= 1;
         count = groups;
     }
     if (threadIdx.x == 0) {
-        const double W = src[0];
+        const double W = bufs[cur][0];
         DevState* st = a.st;
         uint32_t err = 0;
         if (W != W) err = ERR_BAD_WEIGHT;
@@ -95,20 +124,45 @@ __global__ __launch_bounds__(1024) void finalize_sum(BuildArgs a) {
         st->cert_min_dH = 0x7FF0000000000000ull;   // +inf
         st->split_fallbacks = 0;
         st->learned_fallbacks = 0;
+        st->sum_done = 0;
     }
 }
 
 // ---------------------------------------------------------------------------
-// 3. PSA+ / plain partition per section.
+// 2. PSA+ / plain partition.  A workgroup of 4 waves owns a tile of T
+//    consecutive sections, processes them one after another with all its
+//    lanes, then takes one look-back for the tile and writes the leftovers.
+#define PSA_THREADS 256
+#define PSA_MAX_LEN 4096
+#define PSA_MAX_TILE 8
+
 struct SecLDS {
     double* w;      // [Ls]: lights from the front, heavies from the back
+    double* carL;   // [Ls/64 + 2]: chunk carries of the light leftover list
+    double* carH;   // [Ls/64 + 2]
     uint16_t* idx;  // [Ls]: local item index
+    uint16_t* map;  // [Ls]: leftover rank -> position (L ranks, then H ranks)
     int len;
     __device__ double& Lw(int i) { return w[i]; }
     __device__ uint16_t& Li(int i) { return idx[i]; }
     __device__ double& Hw(int j) { return w[len - 1 - j]; }
     __device__ uint16_t& Hi(int j) { return idx[len - 1 - j]; }
 };
+
+__host__ __device__ inline int64_t psa_carry_slots(int64_t Ls_max) { return Ls_max / 64 + 2; }
+
+__device__ __forceinline__ SecLDS section_lds(const BuildArgs& a, double* lds_raw, int q) {
+    char* base = reinterpret_cast<char*>(lds_raw) + (size_t)q * a.lds_region;
+    const int64_t nc = psa_carry_slots(a.Ls_max);
+    SecLDS L;
+    L.w = reinterpret_cast<double*>(base);
+    L.carL = L.w + a.Ls_max;
+    L.carH = L.carL + nc;
+    L.idx = reinterpret_cast<uint16_t*>(L.carH + nc);
+    L.map = L.idx + a.Ls_max;
+    L.len = 0;
+    return L;
+}
 
 template <bool AOS>
 __device__ __forceinline__ void store_row(const BuildArgs& a, int64_t item, double rw, int32_t alias) {
@@ -124,313 +178,379 @@ __device__ __forceinline__ void store_row(const BuildArgs& a, int64_t item, doub
     }
 }
 
-// Rank r of the section's leftover list -> LDS position, given per-chain
-// (offset, first position) held by lanes t < nch.
-__device__ __forceinline__ int leftover_pos(int r, int off, int first, int nch) {
-    int t = 0;
-    for (int tt = 1; tt < nch; ++tt)
-        if (__shfl(off, tt, 64) <= r) t = tt;
-    return __shfl(first, t, 64) + (r - __shfl(off, t, 64));
-}
-
-// Canonical local scan aggregate (chunks of 64, Kogge-Stone, carries chained).
-template <bool HEAVY>
-__device__ double local_aggregate(SecLDS& L, int m, int off, int first, int nch) {
-    double carry = 0.0;
-    for (int r0 = 0; r0 < m; r0 += 64) {
-        const int r = r0 + threadIdx.x;
-        double v = 0.0;
-        const int pos = leftover_pos(r < m ? r : 0, off, first, nch);  // wave-uniform shuffles
-        if (r < m) v = HEAVY ? L.Hw(pos) : L.Lw(pos);
-        const double inc = wave_ks_inclusive(v);
-        carry = carry + shfl_d(inc, 63);
-    }
-    return carry;
-}
-
-// Writes the section's leftover list with global prefixes P + local, and
-// returns the lane-local extreme increment for the split certificate.
-template <bool HEAVY>
-__device__ double write_leftovers(const BuildArgs& a, SecLDS& L, int64_t start, int m, int off, int first,
-                                  int nch, int64_t goff, double P, double agg) {
-    int32_t* oi = HEAVY ? a.Hi : a.Li;
-    double* ow = HEAVY ? a.Hw : a.Lw;
-    double* op = HEAVY ? a.Hp : a.Lp;
-    double ext = HEAVY ? __builtin_inf() : 0.0;
-    double carry = 0.0;
-    const int lane = threadIdx.x;
-    for (int r0 = 0; r0 < m; r0 += 64) {
-        const int r = r0 + lane;
-        double v = 0.0;
-        int pos = leftover_pos(r < m ? r : 0, off, first, nch);
-        if (r < m) v = HEAVY ? L.Hw(pos) : L.Lw(pos);
-        const double inc = wave_ks_inclusive(v);
-        const double prev = shfl_up_d(inc, 1);
-        const double local = carry + (lane == 0 ? 0.0 : prev);
-        const double pref = P + local;
-        const double total = shfl_d(inc, 63);
-        const double carry_next = carry + total;
-        double succ = __shfl_down(pref, 1, 64);
-        if (lane == 63) succ = P + carry_next;
-        if (r == m - 1) succ = P + agg;
-        if (r < m) {
-            const int item = (int)(HEAVY ? L.Hi(pos) : L.Li(pos));
-            oi[goff + r] = (int32_t)(start + item);
-            ow[goff + r] = v;
-            op[goff + r] = pref;
-            const double d = succ - pref;
-            ext = HEAVY ? fmin(ext, d) : fmax(ext, d);
-        }
-        carry = carry_next;
-    }
-    return ext;
-}
-
-// Look-back: returns the exclusive prefix of section s (packed counts and the
-// two FP64 prefixes), folding from the nearest inclusive predecessor left to
-// right so the bits equal the chained definition P_{s+1} = P_s + agg_s.
 struct Prefix {
-    uint64_t c;
+    uint64_t c;  // packed counts: lights | heavies << 32
     double l, h;
 };
 
-__device__ Prefix look_back(const BuildArgs& a, int64_t s, bool& timeout) {
-    const int lane = threadIdx.x;
-    const uint64_t* aggC = a.slots + 0 * a.G;
-    const uint64_t* aggL = a.slots + 1 * a.G;
+// Look-back over tiles: the exclusive prefix of the tile's first section.
+// Phase 1 finds the nearest predecessor tile with a published inclusive value
+// (moving back window by window; tile -1 is a virtual inclusive 0).  Phase 2
+// folds the section aggregates of the tiles after it, left to right.
+__device__ Prefix look_back_tile(const BuildArgs& a, int64_t tile, int T, int lane, bool& timeout) {
+    const uint64_t* aggC = a.slots;
+    const uint64_t* aggL = a.slots + a.G;
     const uint64_t* aggH = a.slots + 2 * a.G;
-    const uint64_t* incC = a.slots + 3 * a.G;
-    const uint64_t* incL = a.slots + 4 * a.G;
-    const uint64_t* incH = a.slots + 5 * a.G;
-    int64_t hi = s - 1;  // newest predecessor of the current window
-    uint32_t spins = 0;
-    // Phase 1: find the nearest predecessor with an inclusive value such that
-    // every later predecessor has its aggregate.
-    int64_t qstar = -1;
-    bool first_window = true;
-    uint64_t vc = 0, vl = 0, vh = 0;  // this lane's values (window of hi)
-    int h = 0;
+    const uint64_t* tC = a.slots + 3 * a.G;
+    const uint64_t* tL = tC + a.Gt;
+    const uint64_t* tH = tL + a.Gt;
+    int64_t hi = tile - 1;
+    Prefix acc{0, 0.0, 0.0};
+    int64_t qstar;
     while (true) {
         const int64_t q = hi - 63 + lane;
-        int status;  // 2 inclusive, 1 aggregate, 0 none
+        uint64_t vc = 0, vl = 0, vh = 0;
+        bool ready;
         if (q < 0) {
-            status = 2; vc = 0; vl = d2u(0.0); vh = d2u(0.0);
+            ready = true;
         } else {
-            vc = slot_load(incC + q); vl = slot_load(incL + q); vh = slot_load(incH + q);
-            if (vc != ATG_NOT_READY && vl != ATG_NOT_READY && vh != ATG_NOT_READY) {
-                status = 2;
-            } else {
-                vc = slot_load(aggC + q); vl = slot_load(aggL + q); vh = slot_load(aggH + q);
-                status = (vc != ATG_NOT_READY && vl != ATG_NOT_READY && vh != ATG_NOT_READY) ? 1 : 0;
+            vc = slot_load(tC + q); vl = slot_load(tL + q); vh = slot_load(tH + q);
+            ready = vc != ATG_NOT_READY && vl != ATG_NOT_READY && vh != ATG_NOT_READY;
+        }
+        const uint64_t mask = __ballot(ready);
+        if (mask != 0ull) {
+            const int h = 63 - __clzll(mask);
+            qstar = hi - 63 + h;
+            if (qstar >= 0) {
+                acc.c = readlane_u64(vc, h);
+                acc.l = u2d(readlane_u64(vl, h));
+                acc.h = u2d(readlane_u64(vh, h));
+            }
+            break;
+        }
+        hi -= 64;
+    }
+    const int64_t first = (qstar + 1) * T, last = tile * T;  // sections to fold: [first, last)
+    for (int64_t base = first; base < last; base += 64) {
+        const int64_t sec = base + lane;
+        const int cnt = (int)min<int64_t>(64, last - base);
+        uint64_t c = 0, l = 0, h = 0;
+        if (lane < cnt) {
+            uint32_t spins = 0;
+            while (true) {
+                c = slot_load(aggC + sec); l = slot_load(aggL + sec); h = slot_load(aggH + sec);
+                if (c != ATG_NOT_READY && l != ATG_NOT_READY && h != ATG_NOT_READY) break;
+                if (++spins > (1u << 22)) { timeout = true; c = 0; l = d2u(0.0); h = d2u(0.0); break; }
+                __builtin_amdgcn_s_sleep(1);
             }
         }
-        const uint64_t incl_mask = __ballot(status == 2);
-        const uint64_t none_mask = __ballot(status == 0);
-        if (incl_mask != 0ull) {
-            h = 63 - __clzll(incl_mask);
-            const uint64_t above = (h == 63) ? 0ull : (none_mask >> (h + 1));
-            if (above == 0ull) { qstar = hi - 63 + h; break; }
-        } else if (none_mask == 0ull) {
-            hi -= 64;
-            first_window = false;
-            continue;
-        }
-        if (++spins > (1u << 22)) { timeout = true; Prefix z{0, 0.0, 0.0}; return z; }
-        __builtin_amdgcn_s_sleep(2);
-    }
-    // Phase 2: fold left to right from q*.
-    Prefix acc;
-    acc.c = __shfl(vc, h, 64);
-    acc.l = u2d(__shfl(vl, h, 64));
-    acc.h = u2d(__shfl(vh, h, 64));
-    if (first_window) {
-        for (int t = h + 1; t < 64; ++t) {
-            acc.c += __shfl(vc, t, 64);
-            acc.l = acc.l + u2d(__shfl(vl, t, 64));
-            acc.h = acc.h + u2d(__shfl(vh, t, 64));
-        }
-        return acc;
-    }
-    // q* lies in an older window: re-read the (already published) aggregates.
-    for (int64_t cur = qstar + 1; cur < s; cur += 64) {
-        const int64_t q = cur + lane;
-        uint64_t c = 0, l = 0, hh = 0;
-        if (q < s) { c = slot_load(aggC + q); l = slot_load(aggL + q); hh = slot_load(aggH + q); }
-        const int cnt = (int)min<int64_t>(64, s - cur);
         for (int t = 0; t < cnt; ++t) {
-            acc.c += __shfl(c, t, 64);
-            acc.l = acc.l + u2d(__shfl(l, t, 64));
-            acc.h = acc.h + u2d(__shfl(hh, t, 64));
+            acc.c += readlane_u64(c, t);
+            acc.l = acc.l + u2d(readlane_u64(l, t));
+            acc.h = acc.h + u2d(readlane_u64(h, t));
         }
     }
     return acc;
 }
 
-template <typename T, bool AOS>
-__global__ __launch_bounds__(64) void psa_sections(BuildArgs a) {
-    extern __shared__ double lds_raw[];
-    DevState* st = a.st;
-    if (st->error) return;
-    const int lane = threadIdx.x;
-    uint32_t sid = 0;
-    if (lane == 0) sid = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int64_t s = (int64_t)__shfl(sid, 0, 64);
-    if (s >= a.G) return;
-    const T* w = (const T*)a.weights;
-    const int64_t start = min(a.n, s * a.Ls), end = min(a.n, (s + 1) * a.Ls);
-    const int len = (int)(end - start);
-    const double W_N = st->W_N;
-    SecLDS L;
-    L.w = lds_raw;
-    L.idx = reinterpret_cast<uint16_t*>(lds_raw + a.Ls);
-    L.len = len;
+// Shared scratch of one workgroup.
+struct PsaShared {
+    uint32_t tile;
+    int nL, nH, mL, mH;
+    int a[64], b[64];                  // per (chunk of 256, wave): counts -> bases
+    int cl[64], ch[64], il[64], jl[64]; // per chain: leftover counts and first positions
+    int ol[64], oh[64];                // per chain: leftover offsets
+    int len[PSA_MAX_TILE], secL[PSA_MAX_TILE], secH[PSA_MAX_TILE];
+    double aggL[PSA_MAX_TILE], aggH[PSA_MAX_TILE];
+    uint64_t pc[PSA_MAX_TILE];
+    double pl[PSA_MAX_TILE], ph[PSA_MAX_TILE];
+    double red[8];
+};
 
-    // Stable partition: lights to the front in index order, heavies to the
-    // back (h[k] at len-1-k) in index order.
-    int cntL = 0, cntH = 0;
-    for (int c0 = 0; c0 < len; c0 += 4 * 64) {
-        double x[4];
+// Partition, chains, leftover maps and chunk scans of one section.
+template <typename T, bool AOS>
+__device__ void psa_section(const BuildArgs& a, PsaShared& sh, SecLDS& L, int64_t start, int len, double W_N,
+                            int tid, int lane, int wave) {
+    const T* w = (const T*)a.weights;
+    const int U = (len + 255) >> 8;
+    const uint64_t lt = lanemask_lt();
+    double x[PSA_MAX_LEN / 256];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int k = c0 + u * 64 + lane;
-            x[u] = (k < len) ? (double)w[start + k] : 0.0;
-        }
+    for (int u = 0; u < PSA_MAX_LEN / 256; ++u) {
+        const int k = (u << 8) + tid;
+        x[u] = (u < U && k < len) ? (double)w[start + k] : 0.0;
+    }
+    // Stable partition: chunk u of 256 items, wave w covers its 64-item slice.
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int k = c0 + u * 64 + lane;
+    for (int u = 0; u < PSA_MAX_LEN / 256; ++u) {
+        if (u < U) {
+            const int k = (u << 8) + tid;
             const bool valid = k < len;
-            const bool light = valid && x[u] <= W_N;
-            const bool heavy = valid && !(x[u] <= W_N);
-            const uint64_t mL = __ballot(light), mH = __ballot(heavy);
-            const uint64_t lt = lanemask_lt();
+            const uint64_t bl = __ballot(valid && x[u] <= W_N);
+            const uint64_t bh = __ballot(valid && !(x[u] <= W_N));
+            if (lane == 0) { sh.a[u * 4 + wave] = __popcll(bl); sh.b[u * 4 + wave] = __popcll(bh); }
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const int va = (lane < 4 * U) ? sh.a[lane] : 0, vb = (lane < 4 * U) ? sh.b[lane] : 0;
+        const int ea = wave_excl_scan_int(va), eb = wave_excl_scan_int(vb);
+        const int ta = wave_sum_int(va), tb = wave_sum_int(vb);
+        sh.a[lane] = ea;
+        sh.b[lane] = eb;
+        if (lane == 0) { sh.nL = ta; sh.nH = tb; }
+    }
+    __syncthreads();
+    L.len = len;
+#pragma unroll
+    for (int u = 0; u < PSA_MAX_LEN / 256; ++u) {
+        if (u < U) {
+            const int k = (u << 8) + tid;
+            const bool valid = k < len;
+            const bool light = valid && x[u] <= W_N, heavy = valid && !(x[u] <= W_N);
+            const uint64_t bl = __ballot(light), bh = __ballot(heavy);
             if (light) {
-                const int pos = cntL + __popcll(mL & lt);
+                const int pos = sh.a[u * 4 + wave] + __popcll(bl & lt);
                 L.Lw(pos) = x[u];
                 L.Li(pos) = (uint16_t)k;
             }
             if (heavy) {
-                const int pos = cntH + __popcll(mH & lt);
+                const int pos = sh.b[u * 4 + wave] + __popcll(bh & lt);
                 L.Hw(pos) = x[u];
                 L.Hi(pos) = (uint16_t)k;
             }
-            cntL += __popcll(mL);
-            cntH += __popcll(mH);
         }
     }
     __syncthreads();
-    const int nL = cntL, nH = cntH;
-
-    // Greedy chains (PsaPlusNormal.cu:51-123) or one chain holding everything.
-    int nch, iLo, iHi, jLo, jHi;
+    const int nL = sh.nL, nH = sh.nH;
+    const bool identity = !a.psa_plus;
+    int nch = 1;
     if (a.psa_plus) {
+        // Greedy chains (PsaPlusNormal.cu:51-123); chain t runs on wave t % 4.
         const int C = a.chains;
         nch = C;
-        const int stepsL = nL / C, stepsH = nH / C;
-        const int t = lane;
-        iLo = t * stepsL; iHi = (t == C - 1) ? nL : (t + 1) * stepsL;
-        jLo = t * stepsH; jHi = (t == C - 1) ? nH : (t + 1) * stepsH;
-        if (t >= C) { iLo = iHi = jLo = jHi = 0; }
-        const bool active = t < C && iHi - iLo > 0 && jHi - jLo > 0;
-        if (active) {
-            int i = iLo, j = jLo;
-            double wc = L.Hw(j);
-            while (true) {
-                const bool ex = wc <= W_N + ATG_EPSILON;
-                if ((ex && j >= jHi - 1) || (!ex && i >= iHi - 1)) break;
-                int item, alias;
-                double rw, next;
-                if (ex) {
-                    item = L.Hi(j); rw = wc; alias = L.Hi(j + 1); next = L.Hw(j + 1); ++j;
-                } else {
-                    item = L.Li(i); rw = L.Lw(i); alias = L.Hi(j); next = L.Lw(i); ++i;
+        const int t = lane * 4 + wave;
+        if (t < C) {
+            const int stepsL = nL / C, stepsH = nH / C;
+            int iLo = t * stepsL, iHi = (t == C - 1) ? nL : (t + 1) * stepsL;
+            int jLo = t * stepsH, jHi = (t == C - 1) ? nH : (t + 1) * stepsH;
+            if (iHi - iLo > 0 && jHi - jLo > 0) {
+                int i = iLo, j = jLo;
+                double wc = L.Hw(j);
+                int hj = L.Hi(j);
+                while (true) {
+                    const bool ex = wc <= W_N + ATG_EPSILON;
+                    if ((ex && j >= jHi - 1) || (!ex && i >= iHi - 1)) break;
+                    int item, alias;
+                    double rw, next;
+                    if (ex) {
+                        const int hn = L.Hi(j + 1);
+                        item = hj; rw = wc; alias = hn; next = L.Hw(j + 1); ++j; hj = hn;
+                    } else {
+                        item = L.Li(i); rw = L.Lw(i); alias = hj; next = rw; ++i;
+                    }
+                    wc += next - W_N;
+                    store_row<AOS>(a, start + item, rw, (int32_t)(start + alias));
                 }
-                wc += next - W_N;
-                store_row<AOS>(a, start + item, rw, (int32_t)(start + alias));
+                // Remainder (PsaPlusNormal.cu:112-123): back to L when its
+                // residual is light and a consumed light slot can hold it,
+                // otherwise back to H in place.
+                if (wc <= W_N + ATG_EPSILON && i > iLo) {
+                    L.Lw(i - 1) = wc;
+                    L.Li(i - 1) = (uint16_t)hj;
+                    iLo = i - 1;
+                    jLo = j + 1;
+                } else {
+                    L.Hw(j) = wc;
+                    iLo = i;
+                    jLo = j;
+                }
             }
-            // Remainder (PsaPlusNormal.cu:112-123): back to L when its residual
-            // is light and the chain consumed a light slot to hold it, else
-            // back to H in place.
-            if (wc <= W_N + ATG_EPSILON && i > iLo) {
-                L.Lw(i - 1) = wc;
-                L.Li(i - 1) = L.Hi(j);
-                iLo = i - 1;
-                jLo = j + 1;
-            } else {
-                L.Hw(j) = wc;
-                iLo = i;
-                jLo = j;
-            }
+            sh.cl[t] = iHi - iLo; sh.ch[t] = jHi - jLo; sh.il[t] = iLo; sh.jl[t] = jLo;
         }
-    } else {
-        nch = 1;
-        iLo = 0; iHi = (lane == 0) ? nL : 0;
-        jLo = 0; jHi = (lane == 0) ? nH : 0;
+    } else if (tid == 0) {
+        sh.cl[0] = nL; sh.ch[0] = nH; sh.il[0] = 0; sh.jl[0] = 0;
     }
     __syncthreads();
-
-    const int cl = iHi - iLo, ch = jHi - jLo;
-    const int offL = wave_excl_scan_int(cl), offH = wave_excl_scan_int(ch);
-    const int mL = wave_sum_int(cl), mH = wave_sum_int(ch);
-
-    const double aggL = local_aggregate<false>(L, mL, offL, iLo, nch);
-    const double aggH = local_aggregate<true>(L, mH, offH, jLo, nch);
-    const uint64_t aggC = (uint64_t)(uint32_t)mL | ((uint64_t)(uint32_t)mH << 32);
-
-    uint64_t* incC = a.slots + 3 * a.G;
-    uint64_t* incLs = a.slots + 4 * a.G;
-    uint64_t* incHs = a.slots + 5 * a.G;
-    Prefix P{0, 0.0, 0.0};
-    if (s > 0) {
-        if (lane == 0) {
-            slot_store(a.slots + 0 * a.G + s, aggC);
-            slot_store(a.slots + 1 * a.G + s, d2u(aggL));
-            slot_store(a.slots + 2 * a.G + s, d2u(aggH));
+    if (wave == 0) {
+        const int vl = (lane < nch) ? sh.cl[lane] : 0, vh = (lane < nch) ? sh.ch[lane] : 0;
+        const int ol = wave_excl_scan_int(vl), oh = wave_excl_scan_int(vh);
+        const int ml = wave_sum_int(vl), mh = wave_sum_int(vh);
+        sh.ol[lane] = ol;
+        sh.oh[lane] = oh;
+        if (lane == 0) { sh.mL = ml; sh.mH = mh; }
+    }
+    __syncthreads();
+    const int mL = sh.mL, mH = sh.mH;
+    if (!identity) {
+        const int t = lane * 4 + wave;
+        if (t < nch) {
+            const int cl = sh.cl[t], ch = sh.ch[t], il = sh.il[t], jl = sh.jl[t], ol = sh.ol[t], oh = sh.oh[t];
+            for (int q = 0; q < cl; ++q) L.map[ol + q] = (uint16_t)(il + q);
+            for (int q = 0; q < ch; ++q) L.map[mL + oh + q] = (uint16_t)(jl + q);
         }
-        bool timeout = false;
-        P = look_back(a, s, timeout);
-        if (timeout && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
     }
-    const uint64_t inclC = P.c + aggC;
-    const double inclL = P.l + aggL, inclH = P.h + aggH;
-    if (lane == 0) {
-        slot_store(incC + s, inclC);
-        slot_store(incLs + s, d2u(inclL));
-        slot_store(incHs + s, d2u(inclH));
+    __syncthreads();
+    // Chunk scans of the two leftover lists: chunk totals in parallel.
+    const int nchL = (mL + 63) >> 6, nchH = (mH + 63) >> 6;
+    for (int c = wave; c < nchL + nchH; c += 4) {
+        const bool heavy = c >= nchL;
+        const int cc = heavy ? c - nchL : c;
+        const int m = heavy ? mH : mL;
+        const int r = (cc << 6) + lane;
+        double v = 0.0;
+        if (r < m) {
+            const int pos = identity ? r : (int)L.map[(heavy ? mL : 0) + r];
+            v = heavy ? L.Hw(pos) : L.Lw(pos);
+        }
+        const double inc = wave_scan(v);
+        if (lane == 63) (heavy ? L.carH : L.carL)[cc] = inc;
     }
+    __syncthreads();
+    // Carries chained left to right (oracle_local_scan).
+    if (tid == 0 || tid == 64) {
+        double* car = (tid == 0) ? L.carL : L.carH;
+        const int nc = (tid == 0) ? nchL : nchH;
+        double carry = 0.0;
+        for (int c = 0; c < nc; ++c) {
+            const double t = car[c];
+            car[c] = carry;
+            carry = carry + t;
+        }
+        car[nc] = carry;
+    }
+    __syncthreads();
+}
 
-    const int64_t goffL = (int64_t)(uint32_t)P.c, goffH = (int64_t)(P.c >> 32);
-    const double dL = write_leftovers<false>(a, L, start, mL, offL, iLo, nch, goffL, P.l, aggL);
-    const double dH = write_leftovers<true>(a, L, start, mH, offH, jLo, nch, goffH, P.h, aggH);
-    double mx = dL, mn = dH;
+// Writes one section's leftover lists with global prefixes and folds the
+// split certificate (max light increment, min heavy increment).
+__device__ void psa_write(const BuildArgs& a, PsaShared& sh, SecLDS& L, int64_t start, int mL, int mH,
+                          uint64_t Pc, double Pl, double Ph, int lane, int wave, double& ext_l, double& ext_h) {
+    const bool identity = !a.psa_plus;
+    const int nchL = (mL + 63) >> 6, nchH = (mH + 63) >> 6;
+    const int64_t goffL = (int64_t)(uint32_t)Pc, goffH = (int64_t)(Pc >> 32);
+    for (int c = wave; c < nchL + nchH; c += 4) {
+        const bool heavy = c >= nchL;
+        const int cc = heavy ? c - nchL : c;
+        const int m = heavy ? mH : mL;
+        const double* car = heavy ? L.carH : L.carL;
+        const double P = heavy ? Ph : Pl;
+        const int r = (cc << 6) + lane;
+        double v = 0.0;
+        int item = 0;
+        if (r < m) {
+            const int pos = identity ? r : (int)L.map[(heavy ? mL : 0) + r];
+            v = heavy ? L.Hw(pos) : L.Lw(pos);
+            item = heavy ? L.Hi(pos) : L.Li(pos);
+        }
+        const double inc = wave_scan(v);
+        const double local = car[cc] + wave_shr1(inc);
+        const double pref = P + local;
+        double succ = __shfl_down(pref, 1, 64);
+        if (lane == 63) succ = P + car[cc + 1];
+        if (r == m - 1) succ = P + car[(m + 63) >> 6];
+        if (r < m) {
+            const int64_t g = (heavy ? goffH : goffL) + r;
+            (heavy ? a.Hi : a.Li)[g] = (int32_t)(start + item);
+            (heavy ? a.Hw : a.Lw)[g] = v;
+            (heavy ? a.Hp : a.Lp)[g] = pref;
+            const double d = succ - pref;
+            if (heavy) ext_h = fmin(ext_h, d); else ext_l = fmax(ext_l, d);
+        }
+    }
+}
+
+template <typename T, bool AOS>
+__global__ __launch_bounds__(PSA_THREADS) void psa_tiles(BuildArgs a) {
+    extern __shared__ double lds_raw[];
+    __shared__ PsaShared sh;
+    DevState* st = a.st;
+    if (st->error) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int TW = a.tile_sections;
+    if (tid == 0) sh.tile = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int64_t tile = sh.tile;
+    const double W_N = st->W_N;
+    for (int q = 0; q < TW; ++q) {
+        const int64_t s = tile * TW + q;
+        if (s >= a.G) {
+            if (tid == 0) sh.len[q] = -1;
+            continue;
+        }
+        const int64_t start = min(a.n, s * a.Ls);
+        const int len = (int)(min(a.n, (s + 1) * a.Ls) - start);
+        SecLDS L = section_lds(a, lds_raw, q);
+        psa_section<T, AOS>(a, sh, L, start, len, W_N, tid, lane, wave);
+        if (tid == 0) {
+            const int mL = sh.mL, mH = sh.mH;
+            const double aggL = L.carL[(mL + 63) >> 6], aggH = L.carH[(mH + 63) >> 6];
+            const uint64_t aggC = (uint64_t)(uint32_t)mL | ((uint64_t)(uint32_t)mH << 32);
+            slot_store(a.slots + s, aggC);
+            slot_store(a.slots + a.G + s, d2u(aggL));
+            slot_store(a.slots + 2 * a.G + s, d2u(aggH));
+            sh.len[q] = len;
+            sh.secL[q] = mL;
+            sh.secH[q] = mH;
+            sh.aggL[q] = aggL;
+            sh.aggH[q] = aggH;
+        }
+        __syncthreads();
+    }
+    if (wave == 0) {
+        bool timeout = false;
+        Prefix P{0, 0.0, 0.0};
+        if (tile > 0) P = look_back_tile(a, tile, TW, lane, timeout);
+        if (lane == 0) {
+            if (timeout) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
+            for (int q = 0; q < TW; ++q) {
+                sh.pc[q] = P.c; sh.pl[q] = P.l; sh.ph[q] = P.h;
+                if (sh.len[q] >= 0) {
+                    P.c += (uint64_t)(uint32_t)sh.secL[q] | ((uint64_t)(uint32_t)sh.secH[q] << 32);
+                    P.l = P.l + sh.aggL[q];
+                    P.h = P.h + sh.aggH[q];
+                }
+            }
+            uint64_t* tC = a.slots + 3 * a.G;
+            slot_store(tC + tile, P.c);
+            slot_store(tC + a.Gt + tile, d2u(P.l));
+            slot_store(tC + 2 * a.Gt + tile, d2u(P.h));
+        }
+    }
+    __syncthreads();
+    double ext_l = 0.0, ext_h = __builtin_inf();
+    bool anyL = false, anyH = false;
+    for (int q = 0; q < TW; ++q) {
+        if (sh.len[q] < 0) continue;
+        const int64_t s = tile * TW + q;
+        const int64_t start = min(a.n, s * a.Ls);
+        SecLDS L = section_lds(a, lds_raw, q);
+        L.len = sh.len[q];
+        const int mL = sh.secL[q], mH = sh.secH[q];
+        anyL |= mL > 0;
+        anyH |= mH > 0;
+        psa_write(a, sh, L, start, mL, mH, sh.pc[q], sh.pl[q], sh.ph[q], lane, wave, ext_l, ext_h);
+        if (s == a.G - 1 && tid == 0) {
+            // Sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
+            const uint64_t inclC = sh.pc[q] + ((uint64_t)(uint32_t)mL | ((uint64_t)(uint32_t)mH << 32));
+            const double inclL = sh.pl[q] + sh.aggL[q], inclH = sh.ph[q] + sh.aggH[q];
+            const int64_t nLr = (int64_t)(uint32_t)inclC, nHr = (int64_t)(inclC >> 32);
+            a.Li[nLr] = (int32_t)(a.n + 1); a.Lw[nLr] = 0.0; a.Lp[nLr] = inclL;
+            a.Hi[nHr] = (int32_t)a.n; a.Hw[nHr] = __builtin_inf(); a.Hp[nHr] = inclH;
+            const int64_t numL = nLr + 1, numH = nHr + 1;
+            st->numL = numL;
+            st->numH = numH;
+            st->degenerate = (nLr == 0 || nHr == 0) ? 1u : 0u;
+            int64_t p = a.p_req;
+            if (a.psa_plus) { p = min(p, (numL + numH) / 10); if (p < 2) p = 2; }
+            st->p = st->degenerate ? 0 : p;
+            st->nToSplit = numL + numH - 2;
+            st->wToSplit = inclL + inclH;
+        }
+    }
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) {
-        mx = fmax(mx, shfl_xor_d(mx, m));
-        mn = fmin(mn, shfl_xor_d(mn, m));
+        ext_l = fmax(ext_l, shfl_xor_d(ext_l, m));
+        ext_h = fmin(ext_h, shfl_xor_d(ext_h, m));
     }
     if (lane == 0) {
-        if (mL > 0) atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)d2u(mx));
-        if (mH > 0) atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)d2u(mn));
-    }
-
-    if (s == a.G - 1 && lane == 0) {
-        // Sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
-        const int64_t nLr = (int64_t)(uint32_t)inclC, nHr = (int64_t)(inclC >> 32);
-        a.Li[nLr] = (int32_t)(a.n + 1); a.Lw[nLr] = 0.0; a.Lp[nLr] = inclL;
-        a.Hi[nHr] = (int32_t)a.n; a.Hw[nHr] = __builtin_inf(); a.Hp[nHr] = inclH;
-        const int64_t numL = nLr + 1, numH = nHr + 1;
-        st->numL = numL;
-        st->numH = numH;
-        st->degenerate = (nLr == 0 || nHr == 0) ? 1u : 0u;
-        int64_t p = a.p_req;
-        if (a.psa_plus) { p = min(p, (numL + numH) / 10); if (p < 2) p = 2; }
-        st->p = st->degenerate ? 0 : p;
-        st->nToSplit = numL + numH - 2;
-        st->wToSplit = inclL + inclH;
+        if (anyL) atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)d2u(ext_l));
+        if (anyH) atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)d2u(ext_h));
     }
 }
 
 // ---------------------------------------------------------------------------
-// 4. Degenerate tables: every leftover row carries its own mass.
+// 3a. Degenerate tables: every leftover row carries its own mass.
 template <bool AOS>
 __global__ void degenerate_fill(BuildArgs a) {
     const DevState* st = a.st;
@@ -445,8 +565,8 @@ __global__ void degenerate_fill(BuildArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// 5. Split: bisection per k with the reference's predicate; 64 probes bound
-//    the loop (the reference spins forever when FOUND is unreachable).
+// 3b. Split: bisection per k with the reference's predicate; 64 probes bound
+//     the loop (the reference spins forever when FOUND is unreachable).
 __device__ __forceinline__ int check_split(int64_t i, int64_t j, double target, const double* Lp, int64_t numL,
                                            const double* Hp, int64_t numH, double& sigma) {
     if (j + 1 >= numH || i - 1 < 0) return 1;  // LEFT
@@ -501,7 +621,7 @@ __global__ __launch_bounds__(256) void split_bisect(BuildArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// 6. Pack one section per thread (PackMethodBasic.cuh:59-126).
+// 3c. Pack one section per thread (PackMethodBasic.cuh:59-126).
 template <bool AOS>
 __global__ __launch_bounds__(256) void pack_basic(BuildArgs a) {
     const DevState* st = a.st;
@@ -533,28 +653,153 @@ __global__ __launch_bounds__(256) void pack_basic(BuildArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// 3d. Pack with LDS staging (the role of PackMethodBasicShared /
+//     PackMethodChunkedShared, PackMethodBasicShared.cu:3-68): one wave packs
+//     chains k0+1 .. k0+64.  Their light items [i_k0, i_kE) and heavy items
+//     [j_k0, j_kE] are two contiguous ranges, copied to LDS with coalesced
+//     loads when they fit; otherwise the wave reads the lists from HBM.
+#define PACK_CAP 2048
+
+struct GlobalLists {
+    const int32_t *Li, *Hi;
+    const double *Lw, *Hw;
+    __device__ int li(int i) const { return Li[i]; }
+    __device__ double lw(int i) const { return Lw[i]; }
+    __device__ int hi(int j) const { return Hi[j]; }
+    __device__ double hw(int j) const { return Hw[j]; }
+};
+struct StagedLists {
+    const int32_t *Li, *Hi;
+    const double *Lw, *Hw;
+    int iA, jA;
+    __device__ int li(int i) const { return Li[i - iA]; }
+    __device__ double lw(int i) const { return Lw[i - iA]; }
+    __device__ int hi(int j) const { return Hi[j - jA]; }
+    __device__ double hw(int j) const { return Hw[j - jA]; }
+};
+
+// One pack chain (PackMethodBasic.cuh:59-126); hj caches the current heavy item.
+template <bool AOS, typename Lists>
+__device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, Split prev, Split cur, double W_N) {
+    const int iUpper = cur.i, jUpper = cur.j;
+    int i = prev.i, j = prev.j;
+    double w = prev.spill;
+    if (prev.spill == 0) w = l.hw(j);
+    int hj = l.hi(j);
+    while (true) {
+        bool ex = w <= W_N + ATG_EPSILON;
+        if (ex && j >= jUpper) ex = false;
+        else if (!ex && i >= iUpper) ex = true;
+        if ((ex && j >= jUpper) || (!ex && i >= iUpper)) break;
+        int32_t item, alias;
+        double rw, next;
+        if (ex) {
+            const int hn = l.hi(j + 1);
+            item = hj; rw = w; alias = hn; next = l.hw(j + 1); ++j; hj = hn;
+        } else {
+            item = l.li(i); rw = l.lw(i); alias = hj; next = rw; ++i;
+        }
+        w += next - W_N;
+        if ((uint32_t)item < (uint32_t)a.n) store_row<AOS>(a, item, rw, alias);
+    }
+}
+
+template <bool AOS>
+__global__ __launch_bounds__(64) void pack_staged(BuildArgs a) {
+    extern __shared__ double lds_raw[];
+    const DevState* st = a.st;
+    if (st->error || st->degenerate) return;
+    const int64_t p = st->p;
+    const int lane = threadIdx.x;
+    const int64_t k0 = (int64_t)blockIdx.x * 64;
+    if (k0 + 1 > p) return;
+    const int64_t kE = min(k0 + 64, p);
+    const Split lo = a.splits[k0], hi = a.splits[kE];
+    const int64_t k = k0 + 1 + lane;
+    Split prev{0, 0, 0.0}, cur{0, 0, 0.0};
+    bool inside = true;
+    if (k <= kE) {
+        prev = a.splits[k - 1];
+        cur = a.splits[k];
+        inside = prev.i >= lo.i && prev.i <= cur.i && cur.i <= hi.i && prev.j >= lo.j && prev.j <= cur.j &&
+                 cur.j <= hi.j;
+    }
+    const int nLu = hi.i - lo.i, nHu = hi.j - lo.j + 1;
+    // Stage only when every chain of the wave stays inside the copied ranges
+    // (always, unless a bisection fell back).
+    const bool staged = __ballot(!inside) == 0ull && nLu >= 0 && nLu <= PACK_CAP && nHu >= 1 && nHu <= PACK_CAP;
+    double* sLw = lds_raw;
+    double* sHw = sLw + PACK_CAP;
+    int32_t* sLi = reinterpret_cast<int32_t*>(sHw + PACK_CAP);
+    int32_t* sHi = sLi + PACK_CAP;
+    if (staged) {
+        const int iA = lo.i, jA = lo.j;
+        for (int t0 = 0; t0 < nLu; t0 += 8 * 64) {
+            double wv[8];
+            int32_t iv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int t = t0 + u * 64 + lane;
+                if (t < nLu) { wv[u] = a.Lw[iA + t]; iv[u] = a.Li[iA + t]; }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int t = t0 + u * 64 + lane;
+                if (t < nLu) { sLw[t] = wv[u]; sLi[t] = iv[u]; }
+            }
+        }
+        for (int t0 = 0; t0 < nHu; t0 += 8 * 64) {
+            double wv[8];
+            int32_t iv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int t = t0 + u * 64 + lane;
+                if (t < nHu) { wv[u] = a.Hw[jA + t]; iv[u] = a.Hi[jA + t]; }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int t = t0 + u * 64 + lane;
+                if (t < nHu) { sHw[t] = wv[u]; sHi[t] = iv[u]; }
+            }
+        }
+        wave_lds_sync();
+    }
+    if (k > kE) return;
+    const double W_N = st->W_N;
+    if (staged) {
+        const StagedLists l{sLi, sHi, sLw, sHw, lo.i, lo.j};
+        pack_chain<AOS>(a, l, prev, cur, W_N);
+    } else {
+        const GlobalLists l{a.Li, a.Hi, a.Lw, a.Hw};
+        pack_chain<AOS>(a, l, prev, cur, W_N);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers.
 static inline unsigned blocks_for(int64_t items, int per) { return (unsigned)((items + per - 1) / per); }
 
 hipError_t launch_sum(const BuildArgs& a, hipStream_t s) {
-    const unsigned g = blocks_for(a.G, 4);
-    if (a.dtype_f32) hipLaunchKernelGGL(sum_sections<float>, dim3(g), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(sum_sections<double>, dim3(g), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(finalize_sum, dim3(1), dim3(1024), 0, s, a);
+    const unsigned g = blocks_for(a.G, 64);
+    if (a.dtype_f32) hipLaunchKernelGGL(sum_sections<float>, dim3(g), dim3(1024), 0, s, a);
+    else hipLaunchKernelGGL(sum_sections<double>, dim3(g), dim3(1024), 0, s, a);
     return hipGetLastError();
 }
 
-size_t psa_lds_bytes(int64_t Ls_max) { return (size_t)Ls_max * 8 + (size_t)Ls_max * 2 + 16; }
+size_t psa_region_bytes(int64_t Ls_max) {
+    return (((size_t)Ls_max * 12 + 16 * (size_t)psa_carry_slots(Ls_max)) + 15) & ~(size_t)15;
+}
+size_t psa_lds_bytes(int64_t Ls_max, int tile_sections) { return psa_region_bytes(Ls_max) * tile_sections; }
 
 hipError_t launch_partition(const BuildArgs& a, hipStream_t s) {
-    const size_t lds = psa_lds_bytes(a.Ls_max);
-    const dim3 g((unsigned)a.G), b(64);
+    const size_t lds = psa_lds_bytes(a.Ls_max, a.tile_sections);
+    const dim3 g((unsigned)a.Gt), b(PSA_THREADS);
     if (a.dtype_f32) {
-        if (a.rows) hipLaunchKernelGGL((psa_sections<float, true>), g, b, lds, s, a);
-        else hipLaunchKernelGGL((psa_sections<float, false>), g, b, lds, s, a);
+        if (a.rows) hipLaunchKernelGGL((psa_tiles<float, true>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((psa_tiles<float, false>), g, b, lds, s, a);
     } else {
-        if (a.rows) hipLaunchKernelGGL((psa_sections<double, true>), g, b, lds, s, a);
-        else hipLaunchKernelGGL((psa_sections<double, false>), g, b, lds, s, a);
+        if (a.rows) hipLaunchKernelGGL((psa_tiles<double, true>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((psa_tiles<double, false>), g, b, lds, s, a);
     }
     return hipGetLastError();
 }
@@ -575,21 +820,20 @@ hipError_t launch_split(const BuildArgs& a, hipStream_t s) {
 
 hipError_t launch_pack(const BuildArgs& a, hipStream_t s) {
     const int64_t kmax = a.p_req;
-    if (a.rows) hipLaunchKernelGGL(pack_basic<true>, dim3(blocks_for(kmax, 256)), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(pack_basic<false>, dim3(blocks_for(kmax, 256)), dim3(256), 0, s, a);
+    const size_t lds = (size_t)PACK_CAP * 24;
+    if (a.rows) hipLaunchKernelGGL(pack_staged<true>, dim3(blocks_for(kmax, 64)), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL(pack_staged<false>, dim3(blocks_for(kmax, 64)), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
-hipError_t set_partition_lds_limit(int64_t Ls_max) {
-    const int bytes = (int)psa_lds_bytes(Ls_max);
-    hipError_t e = hipSuccess;
-    e = hipFuncSetAttribute((const void*)psa_sections<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)psa_sections<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)psa_sections<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)psa_sections<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+hipError_t set_partition_lds_limit(size_t bytes) {
+    const void* fns[] = {(const void*)psa_tiles<double, true>, (const void*)psa_tiles<double, false>,
+                         (const void*)psa_tiles<float, true>, (const void*)psa_tiles<float, false>};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace atg

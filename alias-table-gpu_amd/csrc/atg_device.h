@@ -68,24 +68,48 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return (l == 0) ? 0ull : (~0ull >> (64 - l));
 }
 
-// Pairwise tree over the 64 lanes (butterfly; commutativity makes every lane
-// hold the same bits as t[m] = t[m] + t[m+w] for w = 1, 2, ..., 32).
-__device__ __forceinline__ double wave_tree_sum(double v) {
-#pragma unroll
-    for (int m = 1; m < ATG_WAVE; m <<= 1) v = v + shfl_xor_d(v, m);
-    return v;
+// DPP move of a double (both 32-bit halves with the same control).  Lanes the
+// control leaves without a source (bound_ctrl) or masks out (row_mask) read 0.
+template <int CTRL, int ROW_MASK, bool BOUND>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, ROW_MASK, 0xF, BOUND);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, ROW_MASK, 0xF, BOUND);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Kogge-Stone inclusive scan, offsets 1, 2, ..., 32.
-__device__ __forceinline__ double wave_ks_inclusive(double v) {
-    const int l = lane_id();
-#pragma unroll
-    for (int off = 1; off < ATG_WAVE; off <<= 1) {
-        const double o = shfl_up_d(v, off);
-        if (l >= off) v = o + v;
-    }
+// Inclusive scan over the wave in the canonical order: Kogge-Stone offsets
+// 1, 2, 4, 8 inside rows of 16 lanes (row_shr), rows 1 and 3 add the last lane
+// of the row before (row_bcast:15), rows 2 and 3 add lane 31 (row_bcast:31).
+// Lane 63 holds the pairwise tree of the 64 inputs.
+__device__ __forceinline__ double wave_scan(double v) {
+    v = v + dpp_d<0x111, 0xF, true>(v);
+    v = v + dpp_d<0x112, 0xF, true>(v);
+    v = v + dpp_d<0x114, 0xF, true>(v);
+    v = v + dpp_d<0x118, 0xF, true>(v);
+    v = v + dpp_d<0x142, 0xA, false>(v);
+    v = v + dpp_d<0x143, 0xC, false>(v);
     return v;
 }
+// Value of the previous lane (0 for lane 0): wave_shr:1.
+__device__ __forceinline__ double wave_shr1(double v) { return dpp_d<0x138, 0xF, true>(v); }
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | lo);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)v, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// Pairwise tree over the 64 lanes (t[m] = t[m] + t[m+w], w = 1, 2, ..., 32),
+// returned to every lane.
+__device__ __forceinline__ double wave_tree_sum(double v) { return readlane_d(wave_scan(v), 63); }
 
 template <typename T>
 __device__ __forceinline__ T wave_sum_int(T v) {

@@ -125,18 +125,32 @@ extern "C" double oracle_weight_sum(const double* w, int64_t n, int64_t section_
 
 /* Canonical exclusive scan of one section's list (replaces CUB
  * DeviceScan::ExclusiveSum at AliasTableSplitGpu.cu:406-430; the CPU
- * exclusivePrefixSum is utils/PrefixSum.cu:3-11).  Chunks of 64 values: a
- * Kogge-Stone inclusive scan (offsets 1..32) inside the chunk, chunk carries
- * chained left to right.  Returns the list aggregate (the final carry). */
+ * exclusivePrefixSum is utils/PrefixSum.cu:3-11).  Chunks of 64 values: an
+ * inclusive scan in the order of a 64-lane wave with four rows of 16 lanes -
+ * Kogge-Stone offsets 1, 2, 4, 8 inside each row, then rows 1 and 3 add the
+ * last lane of the row before them, then rows 2 and 3 add lane 31 - and chunk
+ * carries chained left to right.  Lane 63 of that scan is the pairwise tree of
+ * the chunk.  Returns the list aggregate (the final carry). */
+static void wave_scan64(double s[64]) {
+    double t[64];
+    for (int off = 1; off < 16; off *= 2) {
+        std::memcpy(t, s, sizeof(t));
+        for (int l = 0; l < 64; ++l)
+            if ((l & 15) >= off) s[l] = t[l - off] + t[l];
+    }
+    std::memcpy(t, s, sizeof(t));
+    for (int l = 16; l < 32; ++l) s[l] = t[15] + t[l];
+    for (int l = 48; l < 64; ++l) s[l] = t[47] + t[l];
+    std::memcpy(t, s, sizeof(t));
+    for (int l = 32; l < 64; ++l) s[l] = t[31] + t[l];
+}
+
 extern "C" double oracle_local_scan(const double* v, int64_t m, double* excl) {
     double carry = 0.0;
     for (int64_t c0 = 0; c0 < m; c0 += 64) {
-        double s[64], t[64];
+        double s[64];
         for (int l = 0; l < 64; ++l) s[l] = (c0 + l < m) ? v[c0 + l] : 0.0;
-        for (int off = 1; off < 64; off *= 2) {
-            std::memcpy(t, s, sizeof(s));
-            for (int l = off; l < 64; ++l) s[l] = t[l - off] + t[l];
-        }
+        wave_scan64(s);
         if (excl) {
             for (int l = 0; l < 64 && c0 + l < m; ++l) excl[c0 + l] = carry + (l == 0 ? 0.0 : s[l - 1]);
         }
