@@ -54,7 +54,7 @@ int device_cus() {
     return cus;
 }
 
-constexpr int64_t kMaxSectionLen = 4096;  // items of one section staged in registers + LDS (PSA_MAX_LEN)
+constexpr int64_t kMaxSectionLen = 2048;  // items of one section staged in registers + LDS (PSA_MAX_LEN)
 constexpr int64_t kMaxN = 0x7FFFFFF0LL;
 
 }  // namespace
@@ -64,8 +64,7 @@ struct atg_table_s {
     atg_dtype dtype = ATG_F64;
     atg_layout layout = ATG_AOS;
     atg_build_opts opts{};
-    int64_t G = 0, Ls = 0, Gt = 0, p_req = 0, tree_half = 0;
-    int tile_sections = 1;
+    int64_t G = 0, Ls = 0, tree_total = 0, p_req = 0, tree_half = 0;
     int device = 0;
     // device memory
     void* d_weights_owned = nullptr;
@@ -101,9 +100,7 @@ struct atg_table_s {
         a.G = G;
         a.Ls = Ls;
         a.Ls_max = Ls;
-        a.Gt = Gt;
-        a.tile_sections = tile_sections;
-        a.lds_region = (int64_t)psa_region_bytes(Ls);
+        a.tree_total = tree_total;
         a.st = d_state;
         a.section_sums = d_section_sums;
         a.tree_scratch = d_tree;
@@ -172,8 +169,6 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     if (o.chains < 1 || o.chains > 64) return fail(ATG_EINVAL, "atg_create: chains=%d outside [1, 64]", o.chains);
     if (o.split_method != ATG_SPLIT_BISECT && o.split_method != ATG_SPLIT_LEARNED)
         return fail(ATG_EINVAL, "atg_create: unknown split method %d", o.split_method);
-    if (o.split_method == ATG_SPLIT_LEARNED)
-        return fail(ATG_ENOTSUP, "atg_create: learned split index not available in this build");
     if (o.learned_segments <= 0) o.learned_segments = 4096;
     if (o.section_size <= 0) o.section_size = std::min<int64_t>(1000, n / 10);
     if (o.section_size < 1) o.section_size = 1;
@@ -191,18 +186,7 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     }
     t->p_req = std::max<int64_t>(2, std::min<int64_t>(o.num_splits, n / 10));
     t->tree_half = (t->G + 63) / 64 + 1;
-    // Sections per partition workgroup: about 80 KB of LDS per workgroup so two
-    // fit a compute unit (160 KB).  Launch shape only; results do not change.
-    if (o.tile_sections < 0 || o.tile_sections > 8)
-        return fail(ATG_EINVAL, "atg_create: tile_sections=%d outside [0, 8]", o.tile_sections);
-    t->tile_sections = o.tile_sections > 0
-                           ? o.tile_sections
-                           : (int)std::max<int64_t>(1, std::min<int64_t>(8, (80 * 1024) / (int64_t)psa_region_bytes(t->Ls)));
-    if (psa_lds_bytes(t->Ls, t->tile_sections) > 160 * 1024) {
-        delete t;
-        return fail(ATG_EINVAL, "atg_create: %d sections of %lld items exceed the LDS", o.tile_sections, (long long)t->Ls);
-    }
-    t->Gt = (t->G + t->tile_sections - 1) / t->tile_sections;
+    t->tree_total = block_tree_total(t->G);
     (void)hipGetDevice(&t->device);
 
     auto cleanup = [&](atg_status st) { atg_destroy(t); return st; };
@@ -216,7 +200,7 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     ALLOC(t->d_state, 1);
     ALLOC(t->d_section_sums, t->G);
     ALLOC(t->d_tree, 2 * t->tree_half);
-    ALLOC(t->d_slots, 6 * t->G);
+    ALLOC(t->d_slots, 3 * t->tree_total + (t->tree_total + 1) / 2 + 1);
     ALLOC(t->d_Li, n + 2);
     ALLOC(t->d_Hi, n + 2);
     ALLOC(t->d_Lw, n + 2);
@@ -234,7 +218,7 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     ALLOC(t->d_checksum, 1);
 #undef ALLOC
     if (hipMemset(t->d_state, 0, sizeof(DevState)) != hipSuccess) return cleanup(fail(ATG_EDEVICE, "memset"));
-    if (set_partition_lds_limit(psa_lds_bytes(t->Ls, t->tile_sections)) != hipSuccess)
+    if (set_partition_lds_limit(psa_lds_bytes(t->Ls)) != hipSuccess)
         return cleanup(fail(ATG_EDEVICE, "partition LDS limit"));
     if (o.record_phases) {
         for (auto& e : t->ev)

@@ -16,9 +16,7 @@ struct BuildArgs {
     int64_t G;            // sections
     int64_t Ls;           // items per section
     int64_t Ls_max;       // LDS capacity of the partition kernel (items per section)
-    int64_t Gt;           // tiles of the partition pass (ceil(G / tile_sections))
-    int32_t tile_sections;// sections (waves) per partition workgroup; launch shape only
-    int64_t lds_region;   // LDS bytes per section in the partition kernel
+    int64_t tree_total;   // blocks of the dyadic section-prefix tree (sum over b of G >> b)
     DevState* st;
     double* section_sums;     // G
     double* tree_scratch;     // 2 * tree_scratch_half
@@ -44,8 +42,8 @@ inline void section_layout(int64_t n, int64_t param, int64_t* G, int64_t* Ls) {
     *Ls = n / *G + 1;
 }
 
-size_t psa_region_bytes(int64_t Ls_max);
-size_t psa_lds_bytes(int64_t Ls_max, int tile_sections);
+size_t psa_lds_bytes(int64_t Ls_max);
+int64_t block_tree_total(int64_t G);
 hipError_t set_partition_lds_limit(size_t bytes);
 hipError_t launch_sum(const BuildArgs& a, hipStream_t s);
 hipError_t launch_partition(const BuildArgs& a, hipStream_t s);

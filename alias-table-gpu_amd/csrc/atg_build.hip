@@ -5,19 +5,18 @@
 //   1. sum_sections : per-section FP64 sums (64 sections per block), weight
 //                     validation, the first level of the canonical 64-ary tree;
 //                     the last block to finish completes the tree -> W, W/N.
-//   2. psa_tiles    : a 4-wave workgroup per tile of T sections, tiles taken
-//                     in ticket order; per section: all loads issued at once,
-//                     stable light/heavy partition into LDS, PSA+ greedy
-//                     chains (spread over the 4 waves) writing finished rows,
+//   2. psa_wave     : one wave per section, sections taken in ticket order:
+//                     all loads issued at once, stable light/heavy partition
+//                     into LDS, PSA+ greedy chains writing finished rows,
 //                     leftover maps, the canonical local scan of the leftover
-//                     weights, a decoupled look-back over tiles that folds the
-//                     predecessors' section aggregates left to right (bit
-//                     identical to the chained prefix P_{s+1} = P_s + agg_s),
-//                     leftover lists + prefix sums, the split certificate and
-//                     the sentinels.
+//                     weights, the section prefix from dyadic block sums that
+//                     the sections complete cooperatively (O(log G) depth, the
+//                     association the oracle defines), leftover lists + prefix
+//                     sums, the split certificate and the sentinels.
 //   3. degenerate_fill (only when one leftover class is empty), split, pack.
 #include "atg_build.h"
 #include "atg_device.h"
+#include "atg_split.h"
 
 namespace atg {
 
@@ -72,10 +71,12 @@ __global__ __launch_bounds__(1024) void sum_sections(BuildArgs a) {
         if (s < a.G) sum = section_sum<T>(w, min(a.n, s * a.Ls), min(a.n, (s + 1) * a.Ls), lane);
         if (lane == 0) sh[idx] = sum;
     }
-    // Reset the look-back slots of the partition pass.
-    const int64_t nslots = 3 * a.G + 3 * a.Gt;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += (int64_t)gridDim.x * blockDim.x)
-        a.slots[i] = ATG_NOT_READY;
+    // Reset the dyadic block sums (3 values per block) and their arrival
+    // counters for the partition pass.
+    const int64_t nvals = 3 * a.tree_total, ncnt = (a.tree_total + 1) / 2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvals + ncnt;
+         i += (int64_t)gridDim.x * blockDim.x)
+        a.slots[i] = (i < nvals) ? ATG_NOT_READY : 0ull;
     __syncthreads();
     if (wave == 0) {
         const double r = wave_tree_sum(sh[lane]);
@@ -129,17 +130,15 @@ __global__ __launch_bounds__(1024) void sum_sections(BuildArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// 2. PSA+ / plain partition.  A workgroup of 4 waves owns a tile of T
-//    consecutive sections, processes them one after another with all its
-//    lanes, then takes one look-back for the tile and writes the leftovers.
-#define PSA_THREADS 256
-#define PSA_MAX_LEN 4096
-#define PSA_MAX_TILE 8
+// 2. PSA+ / plain partition: one wave per section, sections taken in ticket
+//    order.  The section prefixes of the leftover sums use the dyadic-block
+//    association of the oracle: each section publishes its aggregate as the
+//    level-0 block, the child that completes a pair second publishes the
+//    parent, and P(s) folds the blocks of s's binary digits from high to low.
+#define PSA_MAX_LEN 2048
 
 struct SecLDS {
     double* w;      // [Ls]: lights from the front, heavies from the back
-    double* carL;   // [Ls/64 + 2]: chunk carries of the light leftover list
-    double* carH;   // [Ls/64 + 2]
     uint16_t* idx;  // [Ls]: local item index
     uint16_t* map;  // [Ls]: leftover rank -> position (L ranks, then H ranks)
     int len;
@@ -148,21 +147,6 @@ struct SecLDS {
     __device__ double& Hw(int j) { return w[len - 1 - j]; }
     __device__ uint16_t& Hi(int j) { return idx[len - 1 - j]; }
 };
-
-__host__ __device__ inline int64_t psa_carry_slots(int64_t Ls_max) { return Ls_max / 64 + 2; }
-
-__device__ __forceinline__ SecLDS section_lds(const BuildArgs& a, double* lds_raw, int q) {
-    char* base = reinterpret_cast<char*>(lds_raw) + (size_t)q * a.lds_region;
-    const int64_t nc = psa_carry_slots(a.Ls_max);
-    SecLDS L;
-    L.w = reinterpret_cast<double*>(base);
-    L.carL = L.w + a.Ls_max;
-    L.carH = L.carL + nc;
-    L.idx = reinterpret_cast<uint16_t*>(L.carH + nc);
-    L.map = L.idx + a.Ls_max;
-    L.len = 0;
-    return L;
-}
 
 template <bool AOS>
 __device__ __forceinline__ void store_row(const BuildArgs& a, int64_t item, double rw, int32_t alias) {
@@ -183,359 +167,249 @@ struct Prefix {
     double l, h;
 };
 
-// Look-back over tiles: the exclusive prefix of the tile's first section.
-// Phase 1 finds the nearest predecessor tile with a published inclusive value
-// (moving back window by window; tile -1 is a virtual inclusive 0).  Phase 2
-// folds the section aggregates of the tiles after it, left to right.
-__device__ Prefix look_back_tile(const BuildArgs& a, int64_t tile, int T, int lane, bool& timeout) {
-    const uint64_t* aggC = a.slots;
-    const uint64_t* aggL = a.slots + a.G;
-    const uint64_t* aggH = a.slots + 2 * a.G;
-    const uint64_t* tC = a.slots + 3 * a.G;
-    const uint64_t* tL = tC + a.Gt;
-    const uint64_t* tH = tL + a.Gt;
-    int64_t hi = tile - 1;
-    Prefix acc{0, 0.0, 0.0};
-    int64_t qstar;
-    while (true) {
-        const int64_t q = hi - 63 + lane;
-        uint64_t vc = 0, vl = 0, vh = 0;
-        bool ready;
-        if (q < 0) {
-            ready = true;
-        } else {
-            vc = slot_load(tC + q); vl = slot_load(tL + q); vh = slot_load(tH + q);
-            ready = vc != ATG_NOT_READY && vl != ATG_NOT_READY && vh != ATG_NOT_READY;
-        }
-        const uint64_t mask = __ballot(ready);
-        if (mask != 0ull) {
-            const int h = 63 - __clzll(mask);
-            qstar = hi - 63 + h;
-            if (qstar >= 0) {
-                acc.c = readlane_u64(vc, h);
-                acc.l = u2d(readlane_u64(vl, h));
-                acc.h = u2d(readlane_u64(vh, h));
-            }
-            break;
-        }
-        hi -= 64;
-    }
-    const int64_t first = (qstar + 1) * T, last = tile * T;  // sections to fold: [first, last)
-    for (int64_t base = first; base < last; base += 64) {
-        const int64_t sec = base + lane;
-        const int cnt = (int)min<int64_t>(64, last - base);
-        uint64_t c = 0, l = 0, h = 0;
-        if (lane < cnt) {
-            uint32_t spins = 0;
-            while (true) {
-                c = slot_load(aggC + sec); l = slot_load(aggL + sec); h = slot_load(aggH + sec);
-                if (c != ATG_NOT_READY && l != ATG_NOT_READY && h != ATG_NOT_READY) break;
-                if (++spins > (1u << 22)) { timeout = true; c = 0; l = d2u(0.0); h = d2u(0.0); break; }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        for (int t = 0; t < cnt; ++t) {
-            acc.c += readlane_u64(c, t);
-            acc.l = acc.l + u2d(readlane_u64(l, t));
-            acc.h = acc.h + u2d(readlane_u64(h, t));
-        }
-    }
-    return acc;
+// Dyadic block sums: level b holds G >> b blocks of 2^b sections.
+__device__ __forceinline__ int64_t level_offset(int64_t G, int b) {
+    int64_t off = 0;
+    for (int q = 0; q < b; ++q) off += G >> q;
+    return off;
 }
 
-// Shared scratch of one workgroup.
-struct PsaShared {
-    uint32_t tile;
-    int nL, nH, mL, mH;
-    int a[64], b[64];                  // per (chunk of 256, wave): counts -> bases
-    int cl[64], ch[64], il[64], jl[64]; // per chain: leftover counts and first positions
-    int ol[64], oh[64];                // per chain: leftover offsets
-    int len[PSA_MAX_TILE], secL[PSA_MAX_TILE], secH[PSA_MAX_TILE];
-    double aggL[PSA_MAX_TILE], aggH[PSA_MAX_TILE];
-    uint64_t pc[PSA_MAX_TILE];
-    double pl[PSA_MAX_TILE], ph[PSA_MAX_TILE];
-    double red[8];
-};
+__device__ __forceinline__ void block_load(const BuildArgs& a, int64_t idx, uint64_t& c, uint64_t& l, uint64_t& h,
+                                           bool& timeout) {
+    const uint64_t* V = a.slots;
+    uint32_t spins = 0;
+    while (true) {
+        c = slot_load(V + idx);
+        l = slot_load(V + a.tree_total + idx);
+        h = slot_load(V + 2 * a.tree_total + idx);
+        if (c != ATG_NOT_READY && l != ATG_NOT_READY && h != ATG_NOT_READY) return;
+        if (++spins > (1u << 22)) { timeout = true; c = 0; l = d2u(0.0); h = d2u(0.0); return; }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
 
-// Partition, chains, leftover maps and chunk scans of one section.
+__device__ __forceinline__ void block_store(const BuildArgs& a, int64_t idx, uint64_t c, double l, double h) {
+    uint64_t* V = a.slots;
+    slot_store(V + idx, c);
+    slot_store(V + a.tree_total + idx, d2u(l));
+    slot_store(V + 2 * a.tree_total + idx, d2u(h));
+}
+
+// Publish section s's aggregate and complete the parents for which it is
+// the second child to arrive (single lane).
+__device__ void block_publish(const BuildArgs& a, int64_t s, uint64_t c, double l, double h, bool& timeout) {
+    const int64_t G = a.G;
+    block_store(a, s, c, l, h);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(a.slots + 3 * a.tree_total);
+    int64_t child = s, off_child = 0, off_parent = G;
+    for (int b = 1; ((int64_t)1 << b) <= G; ++b) {
+        const int64_t parent = child >> 1;
+        if (((parent + 1) << b) > G) break;  // incomplete block: never needed
+        const uint32_t old = __hip_atomic_fetch_add(cnt + off_parent + parent, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) break;  // the sibling completes this block
+        uint64_t lc, ll, lh, rc, rl, rh;
+        block_load(a, off_child + 2 * parent, lc, ll, lh, timeout);
+        block_load(a, off_child + 2 * parent + 1, rc, rl, rh, timeout);
+        c = lc + rc;
+        l = u2d(ll) + u2d(rl);
+        h = u2d(lh) + u2d(rh);
+        block_store(a, off_parent + parent, c, l, h);
+        child = parent;
+        off_child = off_parent;
+        off_parent += G >> b;
+    }
+}
+
+// P(s): lane b loads the block of bit b of s, then the blocks are folded from
+// the highest bit to the lowest.
+__device__ Prefix block_prefix(const BuildArgs& a, int64_t s, int lane, bool& timeout) {
+    uint64_t c = 0, l = d2u(0.0), h = d2u(0.0);
+    const bool set = lane < 62 && ((s >> lane) & 1);
+    if (set) {
+        const int64_t pos = (s >> (lane + 1)) << (lane + 1);
+        block_load(a, level_offset(a.G, lane) + (pos >> lane), c, l, h, timeout);
+    }
+    const uint64_t bits = __ballot(set);
+    Prefix P{0, 0.0, 0.0};
+    for (int b = 63; b >= 0; --b) {
+        if (!((bits >> b) & 1)) continue;
+        P.c += readlane_u64(c, b);
+        P.l = P.l + u2d(readlane_u64(l, b));
+        P.h = P.h + u2d(readlane_u64(h, b));
+    }
+    return P;
+}
+
 template <typename T, bool AOS>
-__device__ void psa_section(const BuildArgs& a, PsaShared& sh, SecLDS& L, int64_t start, int len, double W_N,
-                            int tid, int lane, int wave) {
+__global__ __launch_bounds__(64) void psa_wave(BuildArgs a) {
+    extern __shared__ double lds_raw[];
+    DevState* st = a.st;
+    if (st->error) return;
+    const int lane = threadIdx.x;
+    uint32_t sid = 0;
+    if (lane == 0) sid = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int64_t s = (int64_t)__shfl(sid, 0, 64);
+    if (s >= a.G) return;
     const T* w = (const T*)a.weights;
-    const int U = (len + 255) >> 8;
-    const uint64_t lt = lanemask_lt();
-    double x[PSA_MAX_LEN / 256];
-#pragma unroll
-    for (int u = 0; u < PSA_MAX_LEN / 256; ++u) {
-        const int k = (u << 8) + tid;
-        x[u] = (u < U && k < len) ? (double)w[start + k] : 0.0;
-    }
-    // Stable partition: chunk u of 256 items, wave w covers its 64-item slice.
-#pragma unroll
-    for (int u = 0; u < PSA_MAX_LEN / 256; ++u) {
-        if (u < U) {
-            const int k = (u << 8) + tid;
-            const bool valid = k < len;
-            const uint64_t bl = __ballot(valid && x[u] <= W_N);
-            const uint64_t bh = __ballot(valid && !(x[u] <= W_N));
-            if (lane == 0) { sh.a[u * 4 + wave] = __popcll(bl); sh.b[u * 4 + wave] = __popcll(bh); }
-        }
-    }
-    __syncthreads();
-    if (wave == 0) {
-        const int va = (lane < 4 * U) ? sh.a[lane] : 0, vb = (lane < 4 * U) ? sh.b[lane] : 0;
-        const int ea = wave_excl_scan_int(va), eb = wave_excl_scan_int(vb);
-        const int ta = wave_sum_int(va), tb = wave_sum_int(vb);
-        sh.a[lane] = ea;
-        sh.b[lane] = eb;
-        if (lane == 0) { sh.nL = ta; sh.nH = tb; }
-    }
-    __syncthreads();
+    const int64_t start = min(a.n, s * a.Ls);
+    const int len = (int)(min(a.n, (s + 1) * a.Ls) - start);
+    const double W_N = st->W_N;
+    SecLDS L;
+    L.w = lds_raw;
+    L.idx = reinterpret_cast<uint16_t*>(lds_raw + a.Ls_max);
+    L.map = L.idx + a.Ls_max;
     L.len = len;
+    const uint64_t lt = lanemask_lt();
+
+    // All loads of the section in flight at once, then the stable partition:
+    // lights to the front in index order, heavies to the back (h[k] at
+    // len-1-k) in index order.
+    double x[PSA_MAX_LEN / 64];
 #pragma unroll
-    for (int u = 0; u < PSA_MAX_LEN / 256; ++u) {
-        if (u < U) {
-            const int k = (u << 8) + tid;
+    for (int u = 0; u < PSA_MAX_LEN / 64; ++u) {
+        const int k = (u << 6) + lane;
+        x[u] = (k < len) ? (double)w[start + k] : 0.0;
+    }
+    int nL = 0, nH = 0;
+#pragma unroll
+    for (int u = 0; u < PSA_MAX_LEN / 64; ++u) {
+        const int k = (u << 6) + lane;
+        if ((u << 6) < len) {
             const bool valid = k < len;
             const bool light = valid && x[u] <= W_N, heavy = valid && !(x[u] <= W_N);
             const uint64_t bl = __ballot(light), bh = __ballot(heavy);
             if (light) {
-                const int pos = sh.a[u * 4 + wave] + __popcll(bl & lt);
+                const int pos = nL + __popcll(bl & lt);
                 L.Lw(pos) = x[u];
                 L.Li(pos) = (uint16_t)k;
             }
             if (heavy) {
-                const int pos = sh.b[u * 4 + wave] + __popcll(bh & lt);
+                const int pos = nH + __popcll(bh & lt);
                 L.Hw(pos) = x[u];
                 L.Hi(pos) = (uint16_t)k;
             }
+            nL += __popcll(bl);
+            nH += __popcll(bh);
         }
     }
-    __syncthreads();
-    const int nL = sh.nL, nH = sh.nH;
-    const bool identity = !a.psa_plus;
-    int nch = 1;
+    wave_lds_sync();
+
+    int iLo = 0, iHi = 0, jLo = 0, jHi = 0;
     if (a.psa_plus) {
-        // Greedy chains (PsaPlusNormal.cu:51-123); chain t runs on wave t % 4.
+        // Greedy chains (PsaPlusNormal.cu:51-123), lane t runs chain t.
         const int C = a.chains;
-        nch = C;
-        const int t = lane * 4 + wave;
+        const int t = lane;
         if (t < C) {
             const int stepsL = nL / C, stepsH = nH / C;
-            int iLo = t * stepsL, iHi = (t == C - 1) ? nL : (t + 1) * stepsL;
-            int jLo = t * stepsH, jHi = (t == C - 1) ? nH : (t + 1) * stepsH;
-            if (iHi - iLo > 0 && jHi - jLo > 0) {
-                int i = iLo, j = jLo;
-                double wc = L.Hw(j);
-                int hj = L.Hi(j);
-                while (true) {
-                    const bool ex = wc <= W_N + ATG_EPSILON;
-                    if ((ex && j >= jHi - 1) || (!ex && i >= iHi - 1)) break;
-                    int item, alias;
-                    double rw, next;
-                    if (ex) {
-                        const int hn = L.Hi(j + 1);
-                        item = hj; rw = wc; alias = hn; next = L.Hw(j + 1); ++j; hj = hn;
-                    } else {
-                        item = L.Li(i); rw = L.Lw(i); alias = hj; next = rw; ++i;
-                    }
-                    wc += next - W_N;
-                    store_row<AOS>(a, start + item, rw, (int32_t)(start + alias));
-                }
-                // Remainder (PsaPlusNormal.cu:112-123): back to L when its
-                // residual is light and a consumed light slot can hold it,
-                // otherwise back to H in place.
-                if (wc <= W_N + ATG_EPSILON && i > iLo) {
-                    L.Lw(i - 1) = wc;
-                    L.Li(i - 1) = (uint16_t)hj;
-                    iLo = i - 1;
-                    jLo = j + 1;
-                } else {
-                    L.Hw(j) = wc;
-                    iLo = i;
-                    jLo = j;
-                }
+            iLo = t * stepsL; iHi = (t == C - 1) ? nL : (t + 1) * stepsL;
+            jLo = t * stepsH; jHi = (t == C - 1) ? nH : (t + 1) * stepsH;
+        }
+        if (t < C && iHi - iLo > 0 && jHi - jLo > 0) {
+            int i = iLo, j = jLo;
+            double wc = L.Hw(j);
+            int hj = L.Hi(j);
+            double lw = L.Lw(i);
+            int li = L.Li(i);
+            const double thr = W_N + ATG_EPSILON;
+            while (true) {
+                const bool ex = wc <= thr;
+                if ((ex && j >= jHi - 1) || (!ex && i >= iHi - 1)) break;
+                const double hw1 = L.Hw(j + 1);
+                const int hi1 = L.Hi(j + 1);
+                const int item = ex ? hj : li;
+                const double rw = ex ? wc : lw;
+                const int alias = ex ? hi1 : hj;
+                const double next = ex ? hw1 : lw;
+                wc += next - W_N;
+                store_row<AOS>(a, start + item, rw, (int32_t)(start + alias));
+                j += ex ? 1 : 0;
+                i += ex ? 0 : 1;
+                hj = ex ? hi1 : hj;
+                lw = L.Lw(i);
+                li = L.Li(i);
             }
-            sh.cl[t] = iHi - iLo; sh.ch[t] = jHi - jLo; sh.il[t] = iLo; sh.jl[t] = jLo;
+            // Remainder (PsaPlusNormal.cu:112-123): back to L when its
+            // residual is light and a consumed light slot can hold it,
+            // otherwise back to H in place.
+            if (wc <= thr && i > iLo) {
+                L.Lw(i - 1) = wc;
+                L.Li(i - 1) = (uint16_t)hj;
+                iLo = i - 1;
+                jLo = j + 1;
+            } else {
+                L.Hw(j) = wc;
+                iLo = i;
+                jLo = j;
+            }
         }
-    } else if (tid == 0) {
-        sh.cl[0] = nL; sh.ch[0] = nH; sh.il[0] = 0; sh.jl[0] = 0;
+    } else if (lane == 0) {
+        iHi = nL;
+        jHi = nH;
     }
-    __syncthreads();
-    if (wave == 0) {
-        const int vl = (lane < nch) ? sh.cl[lane] : 0, vh = (lane < nch) ? sh.ch[lane] : 0;
-        const int ol = wave_excl_scan_int(vl), oh = wave_excl_scan_int(vh);
-        const int ml = wave_sum_int(vl), mh = wave_sum_int(vh);
-        sh.ol[lane] = ol;
-        sh.oh[lane] = oh;
-        if (lane == 0) { sh.mL = ml; sh.mH = mh; }
-    }
-    __syncthreads();
-    const int mL = sh.mL, mH = sh.mH;
-    if (!identity) {
-        const int t = lane * 4 + wave;
-        if (t < nch) {
-            const int cl = sh.cl[t], ch = sh.ch[t], il = sh.il[t], jl = sh.jl[t], ol = sh.ol[t], oh = sh.oh[t];
-            for (int q = 0; q < cl; ++q) L.map[ol + q] = (uint16_t)(il + q);
-            for (int q = 0; q < ch; ++q) L.map[mL + oh + q] = (uint16_t)(jl + q);
-        }
-    }
-    __syncthreads();
-    // Chunk scans of the two leftover lists: chunk totals in parallel.
-    const int nchL = (mL + 63) >> 6, nchH = (mH + 63) >> 6;
-    for (int c = wave; c < nchL + nchH; c += 4) {
-        const bool heavy = c >= nchL;
-        const int cc = heavy ? c - nchL : c;
-        const int m = heavy ? mH : mL;
-        const int r = (cc << 6) + lane;
-        double v = 0.0;
-        if (r < m) {
-            const int pos = identity ? r : (int)L.map[(heavy ? mL : 0) + r];
-            v = heavy ? L.Hw(pos) : L.Lw(pos);
-        }
-        const double inc = wave_scan(v);
-        if (lane == 63) (heavy ? L.carH : L.carL)[cc] = inc;
-    }
-    __syncthreads();
-    // Carries chained left to right (oracle_local_scan).
-    if (tid == 0 || tid == 64) {
-        double* car = (tid == 0) ? L.carL : L.carH;
-        const int nc = (tid == 0) ? nchL : nchH;
-        double carry = 0.0;
-        for (int c = 0; c < nc; ++c) {
-            const double t = car[c];
-            car[c] = carry;
-            carry = carry + t;
-        }
-        car[nc] = carry;
-    }
-    __syncthreads();
-}
-
-// Writes one section's leftover lists with global prefixes and folds the
-// split certificate (max light increment, min heavy increment).
-__device__ void psa_write(const BuildArgs& a, PsaShared& sh, SecLDS& L, int64_t start, int mL, int mH,
-                          uint64_t Pc, double Pl, double Ph, int lane, int wave, double& ext_l, double& ext_h) {
+    const int cl = iHi - iLo, ch = jHi - jLo;
+    const int offL = wave_excl_scan_int(cl), offH = wave_excl_scan_int(ch);
+    const int mL = wave_sum_int(cl), mH = wave_sum_int(ch);
     const bool identity = !a.psa_plus;
-    const int nchL = (mL + 63) >> 6, nchH = (mH + 63) >> 6;
-    const int64_t goffL = (int64_t)(uint32_t)Pc, goffH = (int64_t)(Pc >> 32);
-    for (int c = wave; c < nchL + nchH; c += 4) {
-        const bool heavy = c >= nchL;
-        const int cc = heavy ? c - nchL : c;
-        const int m = heavy ? mH : mL;
-        const double* car = heavy ? L.carH : L.carL;
-        const double P = heavy ? Ph : Pl;
-        const int r = (cc << 6) + lane;
-        double v = 0.0;
-        int item = 0;
-        if (r < m) {
-            const int pos = identity ? r : (int)L.map[(heavy ? mL : 0) + r];
-            v = heavy ? L.Hw(pos) : L.Lw(pos);
-            item = heavy ? L.Hi(pos) : L.Li(pos);
-        }
-        const double inc = wave_scan(v);
-        const double local = car[cc] + wave_shr1(inc);
-        const double pref = P + local;
-        double succ = __shfl_down(pref, 1, 64);
-        if (lane == 63) succ = P + car[cc + 1];
-        if (r == m - 1) succ = P + car[(m + 63) >> 6];
-        if (r < m) {
-            const int64_t g = (heavy ? goffH : goffL) + r;
-            (heavy ? a.Hi : a.Li)[g] = (int32_t)(start + item);
-            (heavy ? a.Hw : a.Lw)[g] = v;
-            (heavy ? a.Hp : a.Lp)[g] = pref;
-            const double d = succ - pref;
-            if (heavy) ext_h = fmin(ext_h, d); else ext_l = fmax(ext_l, d);
-        }
+    if (!identity) {
+        for (int q = 0; q < cl; ++q) L.map[offL + q] = (uint16_t)(iLo + q);
+        for (int q = 0; q < ch; ++q) L.map[mL + offH + q] = (uint16_t)(jLo + q);
     }
-}
+    wave_lds_sync();
 
-template <typename T, bool AOS>
-__global__ __launch_bounds__(PSA_THREADS) void psa_tiles(BuildArgs a) {
-    extern __shared__ double lds_raw[];
-    __shared__ PsaShared sh;
-    DevState* st = a.st;
-    if (st->error) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int TW = a.tile_sections;
-    if (tid == 0) sh.tile = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int64_t tile = sh.tile;
-    const double W_N = st->W_N;
-    for (int q = 0; q < TW; ++q) {
-        const int64_t s = tile * TW + q;
-        if (s >= a.G) {
-            if (tid == 0) sh.len[q] = -1;
-            continue;
-        }
-        const int64_t start = min(a.n, s * a.Ls);
-        const int len = (int)(min(a.n, (s + 1) * a.Ls) - start);
-        SecLDS L = section_lds(a, lds_raw, q);
-        psa_section<T, AOS>(a, sh, L, start, len, W_N, tid, lane, wave);
-        if (tid == 0) {
-            const int mL = sh.mL, mH = sh.mH;
-            const double aggL = L.carL[(mL + 63) >> 6], aggH = L.carH[(mH + 63) >> 6];
-            const uint64_t aggC = (uint64_t)(uint32_t)mL | ((uint64_t)(uint32_t)mH << 32);
-            slot_store(a.slots + s, aggC);
-            slot_store(a.slots + a.G + s, d2u(aggL));
-            slot_store(a.slots + 2 * a.G + s, d2u(aggH));
-            sh.len[q] = len;
-            sh.secL[q] = mL;
-            sh.secH[q] = mH;
-            sh.aggL[q] = aggL;
-            sh.aggH[q] = aggH;
-        }
-        __syncthreads();
+    // Canonical local scans (chunks of 64, wave scan, carries chained): aggregates.
+    double aggL = 0.0, aggH = 0.0;
+    for (int r0 = 0; r0 < mL; r0 += 64) {
+        const int r = r0 + lane;
+        const double v = (r < mL) ? L.Lw(identity ? r : (int)L.map[r]) : 0.0;
+        aggL = aggL + readlane_d(wave_scan(v), 63);
     }
-    if (wave == 0) {
-        bool timeout = false;
-        Prefix P{0, 0.0, 0.0};
-        if (tile > 0) P = look_back_tile(a, tile, TW, lane, timeout);
-        if (lane == 0) {
-            if (timeout) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
-            for (int q = 0; q < TW; ++q) {
-                sh.pc[q] = P.c; sh.pl[q] = P.l; sh.ph[q] = P.h;
-                if (sh.len[q] >= 0) {
-                    P.c += (uint64_t)(uint32_t)sh.secL[q] | ((uint64_t)(uint32_t)sh.secH[q] << 32);
-                    P.l = P.l + sh.aggL[q];
-                    P.h = P.h + sh.aggH[q];
-                }
-            }
-            uint64_t* tC = a.slots + 3 * a.G;
-            slot_store(tC + tile, P.c);
-            slot_store(tC + a.Gt + tile, d2u(P.l));
-            slot_store(tC + 2 * a.Gt + tile, d2u(P.h));
-        }
+    for (int r0 = 0; r0 < mH; r0 += 64) {
+        const int r = r0 + lane;
+        const double v = (r < mH) ? L.Hw(identity ? r : (int)L.map[mL + r]) : 0.0;
+        aggH = aggH + readlane_d(wave_scan(v), 63);
     }
-    __syncthreads();
+    const uint64_t aggC = (uint64_t)(uint32_t)mL | ((uint64_t)(uint32_t)mH << 32);
+    bool timeout = false;
+    if (lane == 0) block_publish(a, s, aggC, aggL, aggH, timeout);
+    const Prefix P = block_prefix(a, s, lane, timeout);
+
+    // Leftover lists with global prefixes P + local, and the split certificate.
+    const int64_t goffL = (int64_t)(uint32_t)P.c, goffH = (int64_t)(P.c >> 32);
     double ext_l = 0.0, ext_h = __builtin_inf();
-    bool anyL = false, anyH = false;
-    for (int q = 0; q < TW; ++q) {
-        if (sh.len[q] < 0) continue;
-        const int64_t s = tile * TW + q;
-        const int64_t start = min(a.n, s * a.Ls);
-        SecLDS L = section_lds(a, lds_raw, q);
-        L.len = sh.len[q];
-        const int mL = sh.secL[q], mH = sh.secH[q];
-        anyL |= mL > 0;
-        anyH |= mH > 0;
-        psa_write(a, sh, L, start, mL, mH, sh.pc[q], sh.pl[q], sh.ph[q], lane, wave, ext_l, ext_h);
-        if (s == a.G - 1 && tid == 0) {
-            // Sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
-            const uint64_t inclC = sh.pc[q] + ((uint64_t)(uint32_t)mL | ((uint64_t)(uint32_t)mH << 32));
-            const double inclL = sh.pl[q] + sh.aggL[q], inclH = sh.ph[q] + sh.aggH[q];
-            const int64_t nLr = (int64_t)(uint32_t)inclC, nHr = (int64_t)(inclC >> 32);
-            a.Li[nLr] = (int32_t)(a.n + 1); a.Lw[nLr] = 0.0; a.Lp[nLr] = inclL;
-            a.Hi[nHr] = (int32_t)a.n; a.Hw[nHr] = __builtin_inf(); a.Hp[nHr] = inclH;
-            const int64_t numL = nLr + 1, numH = nHr + 1;
-            st->numL = numL;
-            st->numH = numH;
-            st->degenerate = (nLr == 0 || nHr == 0) ? 1u : 0u;
-            int64_t p = a.p_req;
-            if (a.psa_plus) { p = min(p, (numL + numH) / 10); if (p < 2) p = 2; }
-            st->p = st->degenerate ? 0 : p;
-            st->nToSplit = numL + numH - 2;
-            st->wToSplit = inclL + inclH;
+    for (int hv = 0; hv < 2; ++hv) {
+        const bool heavy = hv == 1;
+        const int m = heavy ? mH : mL;
+        const double Pv = heavy ? P.h : P.l, agg = heavy ? aggH : aggL;
+        int32_t* oi = heavy ? a.Hi : a.Li;
+        double* ow = heavy ? a.Hw : a.Lw;
+        double* op = heavy ? a.Hp : a.Lp;
+        const int64_t goff = heavy ? goffH : goffL;
+        double carry = 0.0;
+        for (int r0 = 0; r0 < m; r0 += 64) {
+            const int r = r0 + lane;
+            double v = 0.0;
+            int item = 0;
+            if (r < m) {
+                const int pos = identity ? r : (int)L.map[(heavy ? mL : 0) + r];
+                v = heavy ? L.Hw(pos) : L.Lw(pos);
+                item = heavy ? L.Hi(pos) : L.Li(pos);
+            }
+            const double inc = wave_scan(v);
+            const double local = carry + wave_shr1(inc);
+            const double pref = Pv + local;
+            const double carry_next = carry + readlane_d(inc, 63);
+            double succ = __shfl_down(pref, 1, 64);
+            if (lane == 63) succ = Pv + carry_next;
+            if (r == m - 1) succ = Pv + agg;
+            if (r < m) {
+                oi[goff + r] = (int32_t)(start + item);
+                ow[goff + r] = v;
+                op[goff + r] = pref;
+                const double d = succ - pref;
+                if (heavy) ext_h = fmin(ext_h, d); else ext_l = fmax(ext_l, d);
+            }
+            carry = carry_next;
         }
     }
 #pragma unroll
@@ -544,9 +418,28 @@ __global__ __launch_bounds__(PSA_THREADS) void psa_tiles(BuildArgs a) {
         ext_h = fmin(ext_h, shfl_xor_d(ext_h, m));
     }
     if (lane == 0) {
-        if (anyL) atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)d2u(ext_l));
-        if (anyH) atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)d2u(ext_h));
+        if (mL > 0) atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)d2u(ext_l));
+        if (mH > 0) atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)d2u(ext_h));
     }
+    if (s == a.G - 1) {
+        // Totals P(G), sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
+        const Prefix Tot = block_prefix(a, a.G, lane, timeout);
+        if (lane == 0) {
+            const int64_t nLr = (int64_t)(uint32_t)Tot.c, nHr = (int64_t)(Tot.c >> 32);
+            a.Li[nLr] = (int32_t)(a.n + 1); a.Lw[nLr] = 0.0; a.Lp[nLr] = Tot.l;
+            a.Hi[nHr] = (int32_t)a.n; a.Hw[nHr] = __builtin_inf(); a.Hp[nHr] = Tot.h;
+            const int64_t numL = nLr + 1, numH = nHr + 1;
+            st->numL = numL;
+            st->numH = numH;
+            st->degenerate = (nLr == 0 || nHr == 0) ? 1u : 0u;
+            int64_t p = a.p_req;
+            if (a.psa_plus) { p = min(p, (numL + numH) / 10); if (p < 2) p = 2; }
+            st->p = st->degenerate ? 0 : p;
+            st->nToSplit = numL + numH - 2;
+            st->wToSplit = Tot.l + Tot.h;
+        }
+    }
+    if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
 }
 
 // ---------------------------------------------------------------------------
@@ -567,43 +460,6 @@ __global__ void degenerate_fill(BuildArgs a) {
 // ---------------------------------------------------------------------------
 // 3b. Split: bisection per k with the reference's predicate; 64 probes bound
 //     the loop (the reference spins forever when FOUND is unreachable).
-__device__ __forceinline__ int check_split(int64_t i, int64_t j, double target, const double* Lp, int64_t numL,
-                                           const double* Hp, int64_t numH, double& sigma) {
-    if (j + 1 >= numH || i - 1 < 0) return 1;  // LEFT
-    if (i >= numL || j < 0) return 2;          // RIGHT
-    const double s = Lp[i] + Hp[j];
-    const double sn = Lp[i - 1] + Hp[j + 1];
-    sigma = s;
-    if (s <= target && sn > target) return 0;  // FOUND
-    if (s <= target) return 2;
-    return 1;
-}
-
-__device__ __forceinline__ bool bisect_split(int64_t k, int64_t p, int64_t nToSplit, double wToSplit,
-                                             const double* Lp, int64_t numL, const double* Hp, const double* Hw,
-                                             int64_t numH, Split& out) {
-    const int64_t n_ = (int64_t)ceil(((double)nToSplit * (double)k) / (double)p);
-    const int64_t b0 = min(n_, numH) - 1;
-    const double target = (wToSplit * (double)n_) / (double)nToSplit;
-    int64_t lo = 0, hi = b0;
-    double sigma = 0.0;
-    for (int it = 0; it < 64; ++it) {
-        const int64_t j = (lo + hi) / 2;
-        const int64_t i = min(n_ - j, numL - 1);
-        const int stt = check_split(i, j, target, Lp, numL, Hp, numH, sigma);
-        if (stt == 0) {
-            out.i = (int32_t)i; out.j = (int32_t)j; out.spill = sigma + Hw[j] - target;
-            return true;
-        }
-        if (stt == 2) lo = j + 1; else hi = j;
-    }
-    int64_t j = min(max<int64_t>(lo, 0), max<int64_t>(b0, 0));
-    int64_t i = min(n_ - j, numL - 1);
-    if (i < 0) i = 0;
-    out.i = (int32_t)i; out.j = (int32_t)j; out.spill = (Lp[i] + Hp[j]) + Hw[j] - target;
-    return false;
-}
-
 __global__ __launch_bounds__(256) void split_bisect(BuildArgs a) {
     DevState* st = a.st;
     if (st->error || st->degenerate) return;
@@ -786,20 +642,23 @@ hipError_t launch_sum(const BuildArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-size_t psa_region_bytes(int64_t Ls_max) {
-    return (((size_t)Ls_max * 12 + 16 * (size_t)psa_carry_slots(Ls_max)) + 15) & ~(size_t)15;
+size_t psa_lds_bytes(int64_t Ls_max) { return (((size_t)Ls_max * 12) + 15) & ~(size_t)15; }
+
+int64_t block_tree_total(int64_t G) {
+    int64_t total = 0;
+    for (int b = 0; ((int64_t)1 << b) <= G; ++b) total += G >> b;
+    return total;
 }
-size_t psa_lds_bytes(int64_t Ls_max, int tile_sections) { return psa_region_bytes(Ls_max) * tile_sections; }
 
 hipError_t launch_partition(const BuildArgs& a, hipStream_t s) {
-    const size_t lds = psa_lds_bytes(a.Ls_max, a.tile_sections);
-    const dim3 g((unsigned)a.Gt), b(PSA_THREADS);
+    const size_t lds = psa_lds_bytes(a.Ls_max);
+    const dim3 g((unsigned)a.G), b(64);
     if (a.dtype_f32) {
-        if (a.rows) hipLaunchKernelGGL((psa_tiles<float, true>), g, b, lds, s, a);
-        else hipLaunchKernelGGL((psa_tiles<float, false>), g, b, lds, s, a);
+        if (a.rows) hipLaunchKernelGGL((psa_wave<float, true>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((psa_wave<float, false>), g, b, lds, s, a);
     } else {
-        if (a.rows) hipLaunchKernelGGL((psa_tiles<double, true>), g, b, lds, s, a);
-        else hipLaunchKernelGGL((psa_tiles<double, false>), g, b, lds, s, a);
+        if (a.rows) hipLaunchKernelGGL((psa_wave<double, true>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((psa_wave<double, false>), g, b, lds, s, a);
     }
     return hipGetLastError();
 }
@@ -827,8 +686,8 @@ hipError_t launch_pack(const BuildArgs& a, hipStream_t s) {
 }
 
 hipError_t set_partition_lds_limit(size_t bytes) {
-    const void* fns[] = {(const void*)psa_tiles<double, true>, (const void*)psa_tiles<double, false>,
-                         (const void*)psa_tiles<float, true>, (const void*)psa_tiles<float, false>};
+    const void* fns[] = {(const void*)psa_wave<double, true>, (const void*)psa_wave<double, false>,
+                         (const void*)psa_wave<float, true>, (const void*)psa_wave<float, false>};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;

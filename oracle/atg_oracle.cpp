@@ -301,6 +301,37 @@ static SectionOut process_section(const double* w, int64_t n, int64_t start, int
     return out;
 }
 
+/* Exclusive section prefixes of the leftover sums (replaces the CUB scan
+ * association, AliasTableSplitGpu.cu:406-430, which is unpinned).  Block sums
+ * over aligned dyadic ranges of sections: T(0,c) = agg_c and
+ * T(b,c) = T(b-1,2c) + T(b-1,2c+1) for complete ranges [c 2^b, (c+1) 2^b) in
+ * [0, G).  P(s) folds, from the highest set bit of s to the lowest, the block
+ * that starts where the previous one ended: P = 0; P = P + T(b, pos >> b);
+ * pos += 2^b.  P(G) is the total.  Depth O(log G), and every P(s) depends only
+ * on sections before s. */
+static void section_prefixes(const std::vector<double>& agg, std::vector<double>& P) {
+    const int64_t G = (int64_t)agg.size();
+    std::vector<std::vector<double>> T;
+    T.push_back(agg);
+    for (int b = 1; ((int64_t)1 << b) <= G; ++b) {
+        const std::vector<double>& prev = T.back();
+        std::vector<double> cur(G >> b);
+        for (int64_t c = 0; c < (int64_t)cur.size(); ++c) cur[c] = prev[2 * c] + prev[2 * c + 1];
+        T.push_back(cur);
+    }
+    P.assign(G + 1, 0.0);
+    for (int64_t s = 0; s <= G; ++s) {
+        double acc = 0.0;
+        int64_t pos = 0;
+        for (int b = 62; b >= 0; --b) {
+            if (!((s >> b) & 1)) continue;
+            acc = acc + T[b][pos >> b];
+            pos += (int64_t)1 << b;
+        }
+        P[s] = acc;
+    }
+}
+
 extern "C" int oracle_build(const double* w, int64_t n, const oracle_opts* o, double* row_w,
                             int32_t* row_alias, int32_t* L_item, double* L_w, double* L_pref,
                             int32_t* H_item, double* H_w, double* H_pref, int32_t* split_i,
@@ -348,11 +379,10 @@ extern "C" int oracle_build(const double* w, int64_t n, const oracle_opts* o, do
     }
     std::vector<int64_t> offL(G + 1), offH(G + 1);
     std::vector<double> PL(G + 1), PH(G + 1);
-    offL[0] = offH[0] = 0; PL[0] = PH[0] = 0.0;
-    for (int64_t s = 0; s < G; ++s) {
-        offL[s + 1] = offL[s] + cntL[s]; offH[s + 1] = offH[s] + cntH[s];
-        PL[s + 1] = PL[s] + aggL[s]; PH[s + 1] = PH[s] + aggH[s];
-    }
+    offL[0] = offH[0] = 0;
+    for (int64_t s = 0; s < G; ++s) { offL[s + 1] = offL[s] + cntL[s]; offH[s + 1] = offH[s] + cntH[s]; }
+    section_prefixes(aggL, PL);
+    section_prefixes(aggH, PH);
     const int64_t nLr = offL[G], nHr = offH[G];
     std::vector<int32_t> Li(nLr + 1), Hi(nHr + 1);
     std::vector<double> Lw(nLr + 1), Hw(nHr + 1), Lp(nLr + 1), Hp(nHr + 1);

@@ -56,14 +56,14 @@ CASES = [
     (1000, "pareto", True, 16, 0, 0),
     (10007, "pareto", True, 16, 0, 0),
     (10007, "exponential", False, 16, 0, 1400),
-    (65536, "pareto", True, 64, 4096, 0),
+    (65536, "pareto", True, 64, 2048, 0),
     (100003, "sine", True, 16, 0, 1400),
     (100003, "powerlaw1_shuffled", True, 8, 2000, 0),
     (505711, "one_light", True, 16, 0, 16576),
     (1 << 20, "unit_uniform", True, 16, 0, 0),
     (1 << 20, "pareto", True, 16, 0, 0),
     (1 << 20, "pareto", False, 16, 0, 1400),
-    (1 << 20, "heavy_mix", True, 32, 4096, 0),
+    (1 << 20, "heavy_mix", True, 32, 2048, 0),
 ]
 
 
@@ -75,6 +75,23 @@ def test_build_matches_oracle(gpu, oracle, n, dist, psa, chains, section, p):
     t = gpu_build(w, psa_plus=psa, chains=chains, section_size=section, num_splits=p_req)
     ref = oracle.build(w, psa_plus=psa, chains=chains, section_size=section, num_splits=p_req)
     assert_same_build(t, ref, n)
+
+
+@pytest.mark.parametrize("n,dist,p,segments", [
+    (1 << 20, "pareto", 65536, 4096), (1 << 20, "heavy_mix", 100000, 4096), (3_000_000, "unit_uniform", 300000, 1024),
+    (100003, "sine", 10000, 4096), (100003, "pareto", 7, 4096), (2_000_003, "zipf_shuffled", 200000, 4096)])
+def test_learned_split_bit_identical(gpu, oracle, n, dist, p, segments):
+    """The learned index returns the bisection's splits: identical tables."""
+    w = atg.generate_weights(dist, n, seed=5, param=1.1 if dist == "pareto" else 0.0)
+    plain = gpu_build(w, num_splits=p)
+    learned = gpu_build(w, num_splits=p, split="learned", learned_segments=segments)
+    ref = oracle.build(w, num_splits=p)
+    assert_same_build(learned, ref, n)
+    for a, b in zip(plain.splits(), learned.splits()):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+    st = learned.stats()
+    assert st["monotone"] == 1
+    assert st["learned_fallbacks"] == 0, st
 
 
 @pytest.mark.parametrize("layout", ["aos", "soa"])
