@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libatg.so")
+UNIT_TESTS = os.path.join(HERE, "cpp", "atg_unit_tests")
 SOURCES = ["atg_build.hip", "atg_sample.hip", "atg_batch.hip", "atg_learned.hip",
            "atg_api.cpp", "atg_generate.cpp"]
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -49,6 +50,12 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    # The restated reference self-tests on the C++ facade (run by a GPU test).
+    cmd = [HIPCC, "-O2", "-std=c++17", "-x", "hip", "--cuda-host-only", os.path.join(HERE, "cpp", "unit_tests.cpp"),
+           "-o", UNIT_TESTS, "-L" + HERE, "-latg", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("unit_tests build failed:\n" + r.stderr)
     return LIB
 
 

@@ -1,0 +1,107 @@
+// The reference's self-tests (UnitTests.cu:3-145) restated on the C++ facade:
+// table correctness for every weight distribution and build variant, the
+// regression case N=505711 / p=16576 / OneLight, and the per-item sampling
+// distribution bound |expected - sampled| < S/N for every sampler.
+// Exit status 0 when everything passes.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+
+#include "atg.hpp"
+
+using namespace atg;
+
+static int failures = 0;
+
+template <Storage S>
+static void expect_build(int64_t n, Variant v, int64_t threads, WeightDistribution d) {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    AliasTableSplitGpu<S> t(n, v, threads, d);
+    const auto t1 = clk::now();
+    t.preBuild();
+    t.build();
+    const auto t2 = clk::now();
+    const bool ok = t.postBuild();
+    const auto t3 = clk::now();
+    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    std::printf("%s %s n=%lld dist=%d max_rel=%.3g (generate %.1f ms, build %.1f ms, verify %.1f ms)\n",
+                ok ? "ok" : "FAILED:", t.name().c_str(), (long long)n, (int)d, t.maxRelError, ms(t0, t1), ms(t1, t2),
+                ms(t2, t3));
+    std::fflush(stdout);
+    if (!ok) ++failures;
+}
+
+// UnitTests::testTableCorrectness (UnitTests.cu:10-94).
+static void testTableCorrectness() {
+    for (int d = 1; d <= MAX_WEIGHT_DISTRIBUTION; ++d) {
+        for (bool psa : {false, true}) {
+            for (atg_split_method sm : {ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED}) {
+                Variant v;
+                v.psaPlus = psa;
+                v.split = sm;
+                expect_build<Storage::ArrayOfStructs>(10000, v, NUMBER_THREADS_AUTO, (WeightDistribution)d);
+                expect_build<Storage::StructOfArrays>(10000, v, NUMBER_THREADS_AUTO, (WeightDistribution)d);
+                if (sm == ATG_SPLIT_BISECT)
+                    expect_build<Storage::ArrayOfStructs>(1000000, v, 1024, (WeightDistribution)d);
+            }
+        }
+    }
+    // Regression: a section group with zero eligible pack lanes (UnitTests.cu:47-49).
+    Variant v;
+    v.psaPlus = false;
+    expect_build<Storage::ArrayOfStructs>(505711, v, 16576, weightDistributionOneLight);
+}
+
+// UnitTests::testSamplingDistribution (UnitTests.cu:96-145).
+static void testSamplingDistribution() {
+    const uint64_t numSamples = 2000000;
+    const int64_t tableSize = 10000;
+    AliasTableSplitGpu<Storage::ArrayOfStructs> aos(tableSize, Variant(), NUMBER_THREADS_AUTO, weightDistributionSine);
+    AliasTableSplitGpu<Storage::StructOfArrays> soa(tableSize, Variant(), NUMBER_THREADS_AUTO, weightDistributionSine);
+    RejectionSampling rej(tableSize, weightDistributionSine);
+    aos.fullBuild();
+    soa.fullBuild();
+    rej.fullBuild();
+    std::vector<double> expected(tableSize);
+    for (int64_t k = 0; k < tableSize; ++k) expected[k] = aos.weights[k] / aos.W * (double)numSamples;  // SamplerExpected
+
+    std::vector<std::unique_ptr<Sampler>> samplers;
+    samplers.emplace_back(new SamplerGpuBasic<Storage::ArrayOfStructs>(aos));
+    samplers.emplace_back(new SamplerGpuSectioned<Storage::ArrayOfStructs>(aos, tableSize / 4));
+    samplers.emplace_back(new SamplerGpuBasic<Storage::StructOfArrays>(soa));
+    samplers.emplace_back(new SamplerGpuSectioned<Storage::StructOfArrays>(soa, tableSize / 4));
+    samplers.emplace_back(new SamplerRejection(rej));
+    for (auto& s : samplers) {
+        const std::vector<int> got = s->getSamplingDistribution(numSamples);
+        long long total = 0;
+        for (int64_t k = 0; k < tableSize; ++k) {
+            total += got[k];
+            if (std::fabs(expected[k] - got[k]) >= (double)(numSamples / tableSize)) {
+                std::printf("FAILED: %s expected %.1f samples but got %d at %lld\n", s->name().c_str(), expected[k],
+                            got[k], (long long)k);
+                ++failures;
+                break;
+            }
+        }
+        if (total != (long long)numSamples) {
+            std::printf("FAILED: %s drew %lld samples, asked %llu\n", s->name().c_str(), total,
+                        (unsigned long long)numSamples);
+            ++failures;
+        }
+        std::printf("%s: %.2f GSamples/s (benchmarkSampling of 1e8)\n", s->name().c_str(), s->benchmarkSampling(100000000));
+    }
+}
+
+int main() {
+    try {
+        testTableCorrectness();
+        testSamplingDistribution();
+    } catch (const Error& e) {
+        std::printf("FAILED with status %d: %s\n", (int)e.status, e.what());
+        return 2;
+    }
+    std::printf("%s (%d failures)\n", failures ? "FAILED" : "OK", failures);
+    return failures ? 1 : 0;
+}

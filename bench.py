@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--section-size", type=int, default=0, help="PSA+ section (0 = reference min(1000, N/10))")
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--num-splits", type=int, default=0, help="p (0 = reference AUTO, 320 per CU)")
+    ap.add_argument("--split", choices=["bisect", "learned"], default="bisect")
+    ap.add_argument("--learned-segments", type=int, default=4096)
     ap.add_argument("--sample-section", type=int, default=4096)
     ap.add_argument("--cpu-n", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -98,7 +100,7 @@ def main():
     w = atg.generate_weights("pareto", n, seed=42, param=1.1)
     stream = torch.cuda.current_stream()
     t = atg.Table(n, psa_plus=True, chains=args.chains, section_size=args.section_size,
-                  num_splits=args.num_splits)
+                  num_splits=args.num_splits, split=args.split, learned_segments=args.learned_segments)
     t.set_weights(w, stream)
     for _ in range(args.warmup):
         t.build(stream)
@@ -148,8 +150,8 @@ def main():
                                "then 1e9 sectioned samples (2^12-row sections) + rejection variant",
                    "n": n, "section_size": st["section_len"], "chains": args.chains, "p": p, "layout": "aos",
                    "psa_plus": True},
-        "roofline": {"bound": "hbm", "kernel": {"partition": "psa_sections", "sum": "sum_sections+finalize_sum",
-                                                "pack": "pack_basic"}[dom],
+        "roofline": {"bound": "hbm", "kernel": {"partition": "psa_wave", "sum": "sum_sections",
+                                                "pack": "pack_staged"}[dom],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": pb[dom], "mean_ms": round(phase_ms[dom], 4)},
