@@ -13,7 +13,7 @@ from ._lib import (ATG_AOS, ATG_F32, ATG_F64, ATG_MEM_DEVICE, ATG_MEM_HOST, ATG_
                    ATG_SAMPLE_SECTIONED, ATG_SOA, ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED, DIST, AtgError,
                    BuildOpts, Stats, check, lib)
 
-__all__ = ["Table", "generate_weights", "default_opts", "lib", "AtgError", "DIST"]
+__all__ = ["Table", "Batch", "generate_weights", "generate_batch", "default_opts", "lib", "AtgError", "DIST"]
 
 
 def _ptr(a):
@@ -46,6 +46,19 @@ def generate_weights(dist, n, seed=42, param=0.0):
     d = DIST[dist] if isinstance(dist, str) else int(dist)
     check(lib().atg_generate_weights(d, int(n), int(seed), float(param), _ptr(w)))
     return w
+
+
+def generate_batch(dist, ntables, lo, hi, seed=42, param=0.0, dtype="f64"):
+    """Offsets (ntables + 1) and concatenated host weights of a batch of tables
+    with sizes log-uniform over [lo, hi] (atg_generate_batch)."""
+    d = DIST[dist] if isinstance(dist, str) else int(dist)
+    offsets = np.empty(int(ntables) + 1, np.int64)
+    check(lib().atg_generate_batch(d, int(ntables), int(lo), int(hi), int(seed), float(param), _ptr(offsets),
+                                   None, ATG_F64))
+    w = np.empty(int(offsets[-1]), np.float32 if dtype == "f32" else np.float64)
+    check(lib().atg_generate_batch(d, int(ntables), int(lo), int(hi), int(seed), float(param), _ptr(offsets),
+                                   _ptr(w), ATG_F32 if dtype == "f32" else ATG_F64))
+    return offsets, w
 
 
 class Table:
@@ -171,3 +184,75 @@ class Table:
         check(lib().atg_rejection_sample(self.h, int(count), int(seed), _ptr(out), _ptr(hist),
                                          C.byref(cs) if checksum else None, _stream(stream)))
         return cs.value if checksum else None
+
+
+class Batch:
+    """Many independent tables built in one persistent launch (atg_batch_*).
+
+    Table t owns weights[offsets[t]:offsets[t+1]] and is built exactly as a
+    `Table` with the same options would build it alone."""
+
+    def __init__(self, offsets, dtype="f64", psa_plus=True, chains=16, section_size=0, num_splits=0,
+                 split="bisect", learned_segments=4096, waves=0):
+        o = default_opts()
+        o.psa_plus = 1 if psa_plus else 0
+        o.chains = int(chains)
+        o.section_size = int(section_size)
+        o.num_splits = int(num_splits)
+        o.split_method = ATG_SPLIT_LEARNED if split == "learned" else ATG_SPLIT_BISECT
+        o.learned_segments = int(learned_segments)
+        o.batch_waves = int(waves)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.ntables = self.offsets.size - 1
+        self.total = int(self.offsets[-1])
+        self.dtype = dtype
+        h = C.c_void_p()
+        check(lib().atg_batch_create(C.byref(h), _ptr(self.offsets), self.ntables,
+                                     ATG_F32 if dtype == "f32" else ATG_F64, C.byref(o)))
+        self.h = h
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            lib().atg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_weights(self, w, stream=None):
+        if hasattr(w, "is_cuda") and w.is_cuda:
+            self._keep = w
+            check(lib().atg_batch_set_weights(self.h, _ptr(w), ATG_MEM_DEVICE, _stream(stream)))
+        else:
+            want = np.float32 if self.dtype == "f32" else np.float64
+            a = np.ascontiguousarray(w, dtype=want)
+            assert a.size == self.total
+            check(lib().atg_batch_set_weights(self.h, _ptr(a), ATG_MEM_HOST, _stream(stream)))
+
+    def build(self, stream=None):
+        check(lib().atg_batch_build(self.h, _stream(stream)))
+
+    def sync(self, stream=None):
+        check(lib().atg_batch_sync(self.h, _stream(stream)))
+
+    def tables(self):
+        """(row weights, aliases) of all tables concatenated, and W per table."""
+        w = np.empty(self.total, np.float64)
+        a = np.empty(self.total, np.int32)
+        W = np.empty(self.ntables, np.float64)
+        check(lib().atg_batch_get_tables(self.h, _ptr(w), _ptr(a), _ptr(W)))
+        return w, a, W
+
+    def sample(self, per_table, seed, out, stream=None):
+        """per_table samples of every table into the device buffer `out`
+        (int32, ntables * per_table; item indices local to each table)."""
+        check(lib().atg_batch_sample(self.h, int(per_table), int(seed), _ptr(out), _stream(stream)))
+
+    def latencies_ns(self):
+        v = np.empty(self.ntables, np.float64)
+        check(lib().atg_batch_latencies(self.h, _ptr(v)))
+        return v

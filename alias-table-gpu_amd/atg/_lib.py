@@ -28,7 +28,7 @@ DIST = {
 class BuildOpts(C.Structure):
     _fields_ = [("psa_plus", C.c_int32), ("chains", C.c_int32), ("section_size", C.c_int64),
                 ("num_splits", C.c_int64), ("split_method", C.c_int32), ("learned_segments", C.c_int32),
-                ("record_phases", C.c_int32), ("reserved", C.c_int32)]
+                ("record_phases", C.c_int32), ("batch_waves", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -65,6 +65,7 @@ SIGNATURES = [
     ("atg_get_rejection_array", _I32, [_P, _P, _U64]),
     ("atg_rejection_sample", _I32, [_P, _U64, _U64, _P, _P, C.POINTER(_U64), _P]),
     ("atg_generate_weights", _I32, [_I32, _I64, _U64, _D, _P]),
+    ("atg_generate_batch", _I32, [_I32, _I64, _I64, _I64, _U64, _D, _P, _P, _I32]),
     ("atg_batch_create", _I32, [C.POINTER(_P), _P, _I64, _I32, C.POINTER(BuildOpts)]),
     ("atg_batch_destroy", None, [_P]),
     ("atg_batch_set_weights", _I32, [_P, _P, _I32, _P]),

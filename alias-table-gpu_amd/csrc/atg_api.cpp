@@ -13,13 +13,16 @@
 
 #include "../../include/atg.h"
 #include "atg_build.h"
+#include "atg_host.h"
 #include "atg_sample.h"
 
 using namespace atg;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
+
+namespace atg {
 
 atg_status fail(atg_status st, const char* fmt, ...) {
     char buf[512];
@@ -31,21 +34,6 @@ atg_status fail(atg_status st, const char* fmt, ...) {
     return st;
 }
 
-#define HIPCHK(call)                                                                         \
-    do {                                                                                     \
-        hipError_t e_ = (call);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(e_ == hipErrorOutOfMemory ? ATG_ENOMEM : ATG_EDEVICE, "%s: %s (%s:%d)", \
-                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                   \
-    } while (0)
-
-template <typename T>
-hipError_t dalloc(T** p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    return hipMalloc((void**)p, count * sizeof(T));
-}
-
 int device_cus() {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 256;
@@ -54,10 +42,15 @@ int device_cus() {
     return cus;
 }
 
-constexpr int64_t kMaxSectionLen = 2048;  // items of one section staged in registers + LDS (PSA_MAX_LEN)
-constexpr int64_t kMaxN = 0x7FFFFFF0LL;
+atg_status state_status(const DevState& st) {
+    if (st.error & ERR_BAD_WEIGHT) return fail(ATG_EINVAL, "weights contain NaN, infinity or negative values");
+    if (st.error & ERR_BAD_SUM) return fail(ATG_EINVAL, "sum of weights is not positive and finite");
+    if (st.error & ERR_SECTION_TOO_LONG) return fail(ATG_EINVAL, "section too long for LDS");
+    if (st.error & ERR_LOOKBACK_TIMEOUT) return fail(ATG_ETIMEOUT, "look-back wait expired");
+    return ATG_OK;
+}
 
-}  // namespace
+}  // namespace atg
 
 struct atg_table_s {
     int64_t n = 0;
@@ -82,7 +75,7 @@ struct atg_table_s {
     int32_t* d_knots = nullptr;
     unsigned long long* d_checksum = nullptr;
     // rejection
-    int32_t* d_A = nullptr;
+    RejEntry* d_A = nullptr;
     uint64_t A_cap = 0;
     uint64_t *d_rej_tot = nullptr, *d_rej_off = nullptr;
     bool rej_built = false;
@@ -295,14 +288,6 @@ atg_status atg_build(atg_table* t, void* stream) {
     return ATG_OK;
 }
 
-static atg_status state_status(const DevState& st) {
-    if (st.error & ERR_BAD_WEIGHT) return fail(ATG_EINVAL, "weights contain NaN, infinity or negative values");
-    if (st.error & ERR_BAD_SUM) return fail(ATG_EINVAL, "sum of weights is not positive and finite");
-    if (st.error & ERR_SECTION_TOO_LONG) return fail(ATG_EINVAL, "section too long for LDS");
-    if (st.error & ERR_LOOKBACK_TIMEOUT) return fail(ATG_ETIMEOUT, "look-back wait expired");
-    return ATG_OK;
-}
-
 atg_status atg_sync(atg_table* t, void* stream) {
     if (!t) return fail(ATG_EINVAL, "atg_sync: NULL table");
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
@@ -355,11 +340,20 @@ atg_status atg_get_table(atg_table* t, double* w, int32_t* al, atg_mem mem, void
         HIPCHK(hipStreamSynchronize(s));
         return ATG_OK;
     }
-    // AoS: strided copies split the 16-byte rows.
-    const hipMemcpyKind k = mem == ATG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    HIPCHK(hipMemcpy2DAsync(w, 8, &t->d_rows[0].weight, sizeof(Row), 8, n, k, s));
-    HIPCHK(hipMemcpy2DAsync(al, 4, &t->d_rows[0].alias, sizeof(Row), 4, n, k, s));
+    // AoS: the 16-byte rows are split by a kernel (device destination) or
+    // copied whole and split on the host.
+    if (mem == ATG_MEM_DEVICE) {
+        HIPCHK(launch_split_rows(t->d_rows, w, al, t->n, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return ATG_OK;
+    }
+    std::vector<Row> rows(n);
+    HIPCHK(hipMemcpyAsync(rows.data(), t->d_rows, n * sizeof(Row), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    for (size_t i = 0; i < n; ++i) {
+        w[i] = rows[i].weight;
+        al[i] = rows[i].alias;
+    }
     return ATG_OK;
 }
 
@@ -549,7 +543,9 @@ atg_status atg_get_rejection_array(atg_table* t, int32_t* A, uint64_t cap) {
     atg_status rc = atg_rejection_size(t, &sz);
     if (rc != ATG_OK) return rc;
     if (cap < sz) return fail(ATG_EINVAL, "atg_get_rejection_array: capacity %llu < %llu", (unsigned long long)cap, (unsigned long long)sz);
-    HIPCHK(hipMemcpy(A, t->d_A, 4 * sz, hipMemcpyDeviceToHost));
+    std::vector<RejEntry> e((size_t)sz);
+    if (sz) HIPCHK(hipMemcpy(e.data(), t->d_A, sizeof(RejEntry) * sz, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < sz; ++i) A[i] = e[i].item;
     return ATG_OK;
 }
 

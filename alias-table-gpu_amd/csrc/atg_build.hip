@@ -14,46 +14,11 @@
 //                     association the oracle defines), leftover lists + prefix
 //                     sums, the split certificate and the sentinels.
 //   3. degenerate_fill (only when one leftover class is empty), split, pack.
-#include "atg_build.h"
-#include "atg_device.h"
-#include "atg_split.h"
+#include <algorithm>
+
+#include "atg_kernels.h"
 
 namespace atg {
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-// ---------------------------------------------------------------------------
-// 1. Section sums.  Lane l of the wave that owns a section adds items
-//    l, l+64, ... in order; the lane sums meet in the pairwise tree.  A NaN,
-//    infinite or negative weight turns the section sum into NaN.
-template <typename T>
-__device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t end, int lane) {
-    double acc = 0.0;
-    bool bad = false;
-    int64_t k = start + lane;
-    for (; k + 7 * 64 < end; k += 8 * 64) {
-        double x[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) x[u] = (double)w[k + u * 64];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            bad |= !(x[u] >= 0.0) || (x[u] == __builtin_inf());
-            acc = acc + x[u];
-        }
-    }
-    for (; k < end; k += 64) {
-        const double x = (double)w[k];
-        bad |= !(x >= 0.0) || (x == __builtin_inf());
-        acc = acc + x;
-    }
-    double sum = wave_tree_sum(acc);
-    if (__ballot(bad) != 0ull) sum = __builtin_nan("");
-    return sum;
-}
 
 // Block b owns sections [64 b, 64 b + 64) (16 waves x 4 sections) and writes
 // the tree of their sums (level 1 of the 64-ary tree).  The last block to
@@ -135,112 +100,6 @@ __global__ __launch_bounds__(1024) void sum_sections(BuildArgs a) {
 //    association of the oracle: each section publishes its aggregate as the
 //    level-0 block, the child that completes a pair second publishes the
 //    parent, and P(s) folds the blocks of s's binary digits from high to low.
-#define PSA_MAX_LEN 2048
-
-struct SecLDS {
-    double* w;      // [Ls]: lights from the front, heavies from the back
-    uint16_t* idx;  // [Ls]: local item index
-    uint16_t* map;  // [Ls]: leftover rank -> position (L ranks, then H ranks)
-    int len;
-    __device__ double& Lw(int i) { return w[i]; }
-    __device__ uint16_t& Li(int i) { return idx[i]; }
-    __device__ double& Hw(int j) { return w[len - 1 - j]; }
-    __device__ uint16_t& Hi(int j) { return idx[len - 1 - j]; }
-};
-
-template <bool AOS>
-__device__ __forceinline__ void store_row(const BuildArgs& a, int64_t item, double rw, int32_t alias) {
-    if (AOS) {
-        Row r;
-        r.weight = rw;
-        r.alias = alias;
-        r.pad = 0;
-        *reinterpret_cast<uint4*>(&a.rows[item]) = *reinterpret_cast<const uint4*>(&r);
-    } else {
-        a.tw[item] = rw;
-        a.ta[item] = alias;
-    }
-}
-
-struct Prefix {
-    uint64_t c;  // packed counts: lights | heavies << 32
-    double l, h;
-};
-
-// Dyadic block sums: level b holds G >> b blocks of 2^b sections.
-__device__ __forceinline__ int64_t level_offset(int64_t G, int b) {
-    int64_t off = 0;
-    for (int q = 0; q < b; ++q) off += G >> q;
-    return off;
-}
-
-__device__ __forceinline__ void block_load(const BuildArgs& a, int64_t idx, uint64_t& c, uint64_t& l, uint64_t& h,
-                                           bool& timeout) {
-    const uint64_t* V = a.slots;
-    uint32_t spins = 0;
-    while (true) {
-        c = slot_load(V + idx);
-        l = slot_load(V + a.tree_total + idx);
-        h = slot_load(V + 2 * a.tree_total + idx);
-        if (c != ATG_NOT_READY && l != ATG_NOT_READY && h != ATG_NOT_READY) return;
-        if (++spins > (1u << 22)) { timeout = true; c = 0; l = d2u(0.0); h = d2u(0.0); return; }
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
-
-__device__ __forceinline__ void block_store(const BuildArgs& a, int64_t idx, uint64_t c, double l, double h) {
-    uint64_t* V = a.slots;
-    slot_store(V + idx, c);
-    slot_store(V + a.tree_total + idx, d2u(l));
-    slot_store(V + 2 * a.tree_total + idx, d2u(h));
-}
-
-// Publish section s's aggregate and complete the parents for which it is
-// the second child to arrive (single lane).
-__device__ void block_publish(const BuildArgs& a, int64_t s, uint64_t c, double l, double h, bool& timeout) {
-    const int64_t G = a.G;
-    block_store(a, s, c, l, h);
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(a.slots + 3 * a.tree_total);
-    int64_t child = s, off_child = 0, off_parent = G;
-    for (int b = 1; ((int64_t)1 << b) <= G; ++b) {
-        const int64_t parent = child >> 1;
-        if (((parent + 1) << b) > G) break;  // incomplete block: never needed
-        const uint32_t old = __hip_atomic_fetch_add(cnt + off_parent + parent, 1u, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) break;  // the sibling completes this block
-        uint64_t lc, ll, lh, rc, rl, rh;
-        block_load(a, off_child + 2 * parent, lc, ll, lh, timeout);
-        block_load(a, off_child + 2 * parent + 1, rc, rl, rh, timeout);
-        c = lc + rc;
-        l = u2d(ll) + u2d(rl);
-        h = u2d(lh) + u2d(rh);
-        block_store(a, off_parent + parent, c, l, h);
-        child = parent;
-        off_child = off_parent;
-        off_parent += G >> b;
-    }
-}
-
-// P(s): lane b loads the block of bit b of s, then the blocks are folded from
-// the highest bit to the lowest.
-__device__ Prefix block_prefix(const BuildArgs& a, int64_t s, int lane, bool& timeout) {
-    uint64_t c = 0, l = d2u(0.0), h = d2u(0.0);
-    const bool set = lane < 62 && ((s >> lane) & 1);
-    if (set) {
-        const int64_t pos = (s >> (lane + 1)) << (lane + 1);
-        block_load(a, level_offset(a.G, lane) + (pos >> lane), c, l, h, timeout);
-    }
-    const uint64_t bits = __ballot(set);
-    Prefix P{0, 0.0, 0.0};
-    for (int b = 63; b >= 0; --b) {
-        if (!((bits >> b) & 1)) continue;
-        P.c += readlane_u64(c, b);
-        P.l = P.l + u2d(readlane_u64(l, b));
-        P.h = P.h + u2d(readlane_u64(h, b));
-    }
-    return P;
-}
-
 template <typename T, bool AOS>
 __global__ __launch_bounds__(64) void psa_wave(BuildArgs a) {
     extern __shared__ double lds_raw[];
@@ -251,195 +110,7 @@ __global__ __launch_bounds__(64) void psa_wave(BuildArgs a) {
     if (lane == 0) sid = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int64_t s = (int64_t)__shfl(sid, 0, 64);
     if (s >= a.G) return;
-    const T* w = (const T*)a.weights;
-    const int64_t start = min(a.n, s * a.Ls);
-    const int len = (int)(min(a.n, (s + 1) * a.Ls) - start);
-    const double W_N = st->W_N;
-    SecLDS L;
-    L.w = lds_raw;
-    L.idx = reinterpret_cast<uint16_t*>(lds_raw + a.Ls_max);
-    L.map = L.idx + a.Ls_max;
-    L.len = len;
-    const uint64_t lt = lanemask_lt();
-
-    // All loads of the section in flight at once, then the stable partition:
-    // lights to the front in index order, heavies to the back (h[k] at
-    // len-1-k) in index order.
-    double x[PSA_MAX_LEN / 64];
-#pragma unroll
-    for (int u = 0; u < PSA_MAX_LEN / 64; ++u) {
-        const int k = (u << 6) + lane;
-        x[u] = (k < len) ? (double)w[start + k] : 0.0;
-    }
-    int nL = 0, nH = 0;
-#pragma unroll
-    for (int u = 0; u < PSA_MAX_LEN / 64; ++u) {
-        const int k = (u << 6) + lane;
-        if ((u << 6) < len) {
-            const bool valid = k < len;
-            const bool light = valid && x[u] <= W_N, heavy = valid && !(x[u] <= W_N);
-            const uint64_t bl = __ballot(light), bh = __ballot(heavy);
-            if (light) {
-                const int pos = nL + __popcll(bl & lt);
-                L.Lw(pos) = x[u];
-                L.Li(pos) = (uint16_t)k;
-            }
-            if (heavy) {
-                const int pos = nH + __popcll(bh & lt);
-                L.Hw(pos) = x[u];
-                L.Hi(pos) = (uint16_t)k;
-            }
-            nL += __popcll(bl);
-            nH += __popcll(bh);
-        }
-    }
-    wave_lds_sync();
-
-    int iLo = 0, iHi = 0, jLo = 0, jHi = 0;
-    if (a.psa_plus) {
-        // Greedy chains (PsaPlusNormal.cu:51-123), lane t runs chain t.
-        const int C = a.chains;
-        const int t = lane;
-        if (t < C) {
-            const int stepsL = nL / C, stepsH = nH / C;
-            iLo = t * stepsL; iHi = (t == C - 1) ? nL : (t + 1) * stepsL;
-            jLo = t * stepsH; jHi = (t == C - 1) ? nH : (t + 1) * stepsH;
-        }
-        if (t < C && iHi - iLo > 0 && jHi - jLo > 0) {
-            int i = iLo, j = jLo;
-            double wc = L.Hw(j);
-            int hj = L.Hi(j);
-            double lw = L.Lw(i);
-            int li = L.Li(i);
-            const double thr = W_N + ATG_EPSILON;
-            while (true) {
-                const bool ex = wc <= thr;
-                if ((ex && j >= jHi - 1) || (!ex && i >= iHi - 1)) break;
-                const double hw1 = L.Hw(j + 1);
-                const int hi1 = L.Hi(j + 1);
-                const int item = ex ? hj : li;
-                const double rw = ex ? wc : lw;
-                const int alias = ex ? hi1 : hj;
-                const double next = ex ? hw1 : lw;
-                wc += next - W_N;
-                store_row<AOS>(a, start + item, rw, (int32_t)(start + alias));
-                j += ex ? 1 : 0;
-                i += ex ? 0 : 1;
-                hj = ex ? hi1 : hj;
-                lw = L.Lw(i);
-                li = L.Li(i);
-            }
-            // Remainder (PsaPlusNormal.cu:112-123): back to L when its
-            // residual is light and a consumed light slot can hold it,
-            // otherwise back to H in place.
-            if (wc <= thr && i > iLo) {
-                L.Lw(i - 1) = wc;
-                L.Li(i - 1) = (uint16_t)hj;
-                iLo = i - 1;
-                jLo = j + 1;
-            } else {
-                L.Hw(j) = wc;
-                iLo = i;
-                jLo = j;
-            }
-        }
-    } else if (lane == 0) {
-        iHi = nL;
-        jHi = nH;
-    }
-    const int cl = iHi - iLo, ch = jHi - jLo;
-    const int offL = wave_excl_scan_int(cl), offH = wave_excl_scan_int(ch);
-    const int mL = wave_sum_int(cl), mH = wave_sum_int(ch);
-    const bool identity = !a.psa_plus;
-    if (!identity) {
-        for (int q = 0; q < cl; ++q) L.map[offL + q] = (uint16_t)(iLo + q);
-        for (int q = 0; q < ch; ++q) L.map[mL + offH + q] = (uint16_t)(jLo + q);
-    }
-    wave_lds_sync();
-
-    // Canonical local scans (chunks of 64, wave scan, carries chained): aggregates.
-    double aggL = 0.0, aggH = 0.0;
-    for (int r0 = 0; r0 < mL; r0 += 64) {
-        const int r = r0 + lane;
-        const double v = (r < mL) ? L.Lw(identity ? r : (int)L.map[r]) : 0.0;
-        aggL = aggL + readlane_d(wave_scan(v), 63);
-    }
-    for (int r0 = 0; r0 < mH; r0 += 64) {
-        const int r = r0 + lane;
-        const double v = (r < mH) ? L.Hw(identity ? r : (int)L.map[mL + r]) : 0.0;
-        aggH = aggH + readlane_d(wave_scan(v), 63);
-    }
-    const uint64_t aggC = (uint64_t)(uint32_t)mL | ((uint64_t)(uint32_t)mH << 32);
-    bool timeout = false;
-    if (lane == 0) block_publish(a, s, aggC, aggL, aggH, timeout);
-    const Prefix P = block_prefix(a, s, lane, timeout);
-
-    // Leftover lists with global prefixes P + local, and the split certificate.
-    const int64_t goffL = (int64_t)(uint32_t)P.c, goffH = (int64_t)(P.c >> 32);
-    double ext_l = 0.0, ext_h = __builtin_inf();
-    for (int hv = 0; hv < 2; ++hv) {
-        const bool heavy = hv == 1;
-        const int m = heavy ? mH : mL;
-        const double Pv = heavy ? P.h : P.l, agg = heavy ? aggH : aggL;
-        int32_t* oi = heavy ? a.Hi : a.Li;
-        double* ow = heavy ? a.Hw : a.Lw;
-        double* op = heavy ? a.Hp : a.Lp;
-        const int64_t goff = heavy ? goffH : goffL;
-        double carry = 0.0;
-        for (int r0 = 0; r0 < m; r0 += 64) {
-            const int r = r0 + lane;
-            double v = 0.0;
-            int item = 0;
-            if (r < m) {
-                const int pos = identity ? r : (int)L.map[(heavy ? mL : 0) + r];
-                v = heavy ? L.Hw(pos) : L.Lw(pos);
-                item = heavy ? L.Hi(pos) : L.Li(pos);
-            }
-            const double inc = wave_scan(v);
-            const double local = carry + wave_shr1(inc);
-            const double pref = Pv + local;
-            const double carry_next = carry + readlane_d(inc, 63);
-            double succ = __shfl_down(pref, 1, 64);
-            if (lane == 63) succ = Pv + carry_next;
-            if (r == m - 1) succ = Pv + agg;
-            if (r < m) {
-                oi[goff + r] = (int32_t)(start + item);
-                ow[goff + r] = v;
-                op[goff + r] = pref;
-                const double d = succ - pref;
-                if (heavy) ext_h = fmin(ext_h, d); else ext_l = fmax(ext_l, d);
-            }
-            carry = carry_next;
-        }
-    }
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        ext_l = fmax(ext_l, shfl_xor_d(ext_l, m));
-        ext_h = fmin(ext_h, shfl_xor_d(ext_h, m));
-    }
-    if (lane == 0) {
-        if (mL > 0) atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)d2u(ext_l));
-        if (mH > 0) atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)d2u(ext_h));
-    }
-    if (s == a.G - 1) {
-        // Totals P(G), sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
-        const Prefix Tot = block_prefix(a, a.G, lane, timeout);
-        if (lane == 0) {
-            const int64_t nLr = (int64_t)(uint32_t)Tot.c, nHr = (int64_t)(Tot.c >> 32);
-            a.Li[nLr] = (int32_t)(a.n + 1); a.Lw[nLr] = 0.0; a.Lp[nLr] = Tot.l;
-            a.Hi[nHr] = (int32_t)a.n; a.Hw[nHr] = __builtin_inf(); a.Hp[nHr] = Tot.h;
-            const int64_t numL = nLr + 1, numH = nHr + 1;
-            st->numL = numL;
-            st->numH = numH;
-            st->degenerate = (nLr == 0 || nHr == 0) ? 1u : 0u;
-            int64_t p = a.p_req;
-            if (a.psa_plus) { p = min(p, (numL + numH) / 10); if (p < 2) p = 2; }
-            st->p = st->degenerate ? 0 : p;
-            st->nToSplit = numL + numH - 2;
-            st->wToSplit = Tot.l + Tot.h;
-        }
-    }
-    if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
+    psa_section<T, AOS>(a, s, lds_raw, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -514,52 +185,6 @@ __global__ __launch_bounds__(256) void pack_basic(BuildArgs a) {
 //     chains k0+1 .. k0+64.  Their light items [i_k0, i_kE) and heavy items
 //     [j_k0, j_kE] are two contiguous ranges, copied to LDS with coalesced
 //     loads when they fit; otherwise the wave reads the lists from HBM.
-#define PACK_CAP 2048
-
-struct GlobalLists {
-    const int32_t *Li, *Hi;
-    const double *Lw, *Hw;
-    __device__ int li(int i) const { return Li[i]; }
-    __device__ double lw(int i) const { return Lw[i]; }
-    __device__ int hi(int j) const { return Hi[j]; }
-    __device__ double hw(int j) const { return Hw[j]; }
-};
-struct StagedLists {
-    const int32_t *Li, *Hi;
-    const double *Lw, *Hw;
-    int iA, jA;
-    __device__ int li(int i) const { return Li[i - iA]; }
-    __device__ double lw(int i) const { return Lw[i - iA]; }
-    __device__ int hi(int j) const { return Hi[j - jA]; }
-    __device__ double hw(int j) const { return Hw[j - jA]; }
-};
-
-// One pack chain (PackMethodBasic.cuh:59-126); hj caches the current heavy item.
-template <bool AOS, typename Lists>
-__device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, Split prev, Split cur, double W_N) {
-    const int iUpper = cur.i, jUpper = cur.j;
-    int i = prev.i, j = prev.j;
-    double w = prev.spill;
-    if (prev.spill == 0) w = l.hw(j);
-    int hj = l.hi(j);
-    while (true) {
-        bool ex = w <= W_N + ATG_EPSILON;
-        if (ex && j >= jUpper) ex = false;
-        else if (!ex && i >= iUpper) ex = true;
-        if ((ex && j >= jUpper) || (!ex && i >= iUpper)) break;
-        int32_t item, alias;
-        double rw, next;
-        if (ex) {
-            const int hn = l.hi(j + 1);
-            item = hj; rw = w; alias = hn; next = l.hw(j + 1); ++j; hj = hn;
-        } else {
-            item = l.li(i); rw = l.lw(i); alias = hj; next = rw; ++i;
-        }
-        w += next - W_N;
-        if ((uint32_t)item < (uint32_t)a.n) store_row<AOS>(a, item, rw, alias);
-    }
-}
-
 template <bool AOS>
 __global__ __launch_bounds__(64) void pack_staged(BuildArgs a) {
     extern __shared__ double lds_raw[];
@@ -682,6 +307,21 @@ hipError_t launch_pack(const BuildArgs& a, hipStream_t s) {
     const size_t lds = (size_t)PACK_CAP * 24;
     if (a.rows) hipLaunchKernelGGL(pack_staged<true>, dim3(blocks_for(kmax, 64)), dim3(64), lds, s, a);
     else hipLaunchKernelGGL(pack_staged<false>, dim3(blocks_for(kmax, 64)), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+// AoS rows -> separate weight and alias arrays (device to device copy-out).
+__global__ __launch_bounds__(256) void split_rows(const Row* rows, double* w, int32_t* al, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint4 v = *reinterpret_cast<const uint4*>(&rows[i]);
+        w[i] = __longlong_as_double(((long long)v.y << 32) | v.x);
+        al[i] = (int32_t)v.z;
+    }
+}
+
+hipError_t launch_split_rows(const Row* rows, double* w, int32_t* al, int64_t n, hipStream_t s) {
+    const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(split_rows, dim3(g > 0 ? g : 1), dim3(256), 0, s, rows, w, al, n);
     return hipGetLastError();
 }
 

@@ -30,10 +30,12 @@ inline double u53(const uint32_t x[4], int half) {
     return (double)(r >> 11) * 0x1.0p-53;
 }
 
+thread_local bool t_serial = false;  // set inside the per-table workers of a batch
+
 template <typename F>
 void parallel_for(int64_t n, F f) {
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (n < 65536) nt = 1;
+    if (n < 65536 || t_serial) nt = 1;
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; ++t)
         th.emplace_back([=] {
@@ -148,5 +150,49 @@ extern "C" atg_status atg_generate_weights(int32_t dist, int64_t n, uint64_t see
         default:
             return ATG_EINVAL;
     }
+    return ATG_OK;
+}
+
+// Many tables: sizes n_t = floor(exp(U(ln lo, ln hi))) (clamped to [lo, hi]),
+// table t drawn with seed + t + 1 from the single-table generator.
+extern "C" atg_status atg_generate_batch(int32_t dist, int64_t ntables, int64_t lo, int64_t hi, uint64_t seed,
+                                         double param, int64_t* offsets, void* out, int32_t out_dtype) {
+    if (!offsets || ntables < 1 || lo < 1 || hi < lo) return ATG_EINVAL;
+    offsets[0] = 0;
+    const double a = std::log((double)lo), b = std::log((double)hi);
+    for (int64_t t = 0; t < ntables; ++t) {
+        uint32_t x[4];
+        philox((uint64_t)t, 0xA0F2u, 0, seed, x);
+        int64_t n = (int64_t)std::floor(std::exp(a + (b - a) * u53(x, 1)));
+        n = std::min(hi, std::max(lo, n));
+        offsets[t + 1] = offsets[t] + n;
+    }
+    if (!out) return ATG_OK;
+    if (out_dtype != ATG_F64 && out_dtype != ATG_F32) return ATG_EINVAL;
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    std::vector<atg_status> rc(nt, ATG_OK);
+    for (unsigned k = 0; k < nt; ++k)
+        th.emplace_back([&, k] {
+            t_serial = true;
+            std::vector<double> tmp;
+            for (int64_t t = k; t < ntables; t += nt) {
+                const int64_t n = offsets[t + 1] - offsets[t];
+                double* dst = out_dtype == ATG_F64 ? (double*)out + offsets[t] : nullptr;
+                if (!dst) {
+                    tmp.resize((size_t)n);
+                    dst = tmp.data();
+                }
+                const atg_status r = atg_generate_weights(dist, n, seed + (uint64_t)t + 1, param, dst);
+                if (r != ATG_OK) rc[k] = r;
+                if (out_dtype == ATG_F32) {
+                    float* f = (float*)out + offsets[t];
+                    for (int64_t i = 0; i < n; ++i) f[i] = (float)dst[i];
+                }
+            }
+        });
+    for (auto& x : th) x.join();
+    for (atg_status r : rc)
+        if (r != ATG_OK) return r;
     return ATG_OK;
 }

@@ -42,7 +42,7 @@ struct RejArgs {
     uint64_t* block_tot;   // nb
     uint64_t* block_off;   // nb
     uint64_t* incl_out;    // n inclusive offsets (nullable)
-    int32_t* A;
+    RejEntry* A;          // |A| <= A_cap slots
     uint64_t A_cap;
     unsigned long long* trials;  // nullable
 };

@@ -241,15 +241,28 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
         }
     __syncthreads();
     // Slots [boff, incl[cnt-1]) of this block, written coalesced; the item of a
-    // slot is found by bisection over the block's inclusive offsets.
+    // slot is found by bisection over the block's inclusive offsets.  The first
+    // slot of a run carries the item's acceptance fraction (FP64; the
+    // reference computes it in float, RejectionSampling.cu:26-28).
     const uint64_t s_end = incl[cnt - 1];
+    const T* w = (const T*)a.weights;
     for (uint64_t s = boff + threadIdx.x; s < s_end; s += REJ_THREADS) {
         int lo = 0, hi = cnt - 1;  // first k with incl[k] > s
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (incl[mid] > s) hi = mid; else lo = mid + 1;
         }
-        a.A[s] = (int32_t)(base + lo);
+        const uint64_t first = lo == 0 ? boff : incl[lo - 1];
+        double accept = 1.0;
+        if (s == first) {
+            const double x = ((double)a.n * (double)w[base + lo]) / W;
+            accept = x - floor(x);
+        }
+        RejEntry e;
+        e.accept = accept;
+        e.item = (int32_t)(base + lo);
+        e.pad = 0;
+        *reinterpret_cast<uint4*>(&a.A[s]) = *reinterpret_cast<const uint4*>(&e);
     }
 }
 
@@ -260,8 +273,6 @@ __global__ __launch_bounds__(256) void rej_sample(RejArgs a, SampleArgs sa) {
     if (a.st->error) return;
     const uint64_t Asz = a.st->rej_size;
     if (Asz == 0 || Asz > a.A_cap) return;
-    const double W = a.st->W;
-    const T* w = (const T*)a.weights;
     uint64_t csum = 0, trials = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < sa.count; g += stride) {
@@ -270,12 +281,12 @@ __global__ __launch_bounds__(256) void rej_sample(RejArgs a, SampleArgs sa) {
             const u32x4 r = philox_at(g, PURPOSE_REJECTION, tr, sa.seed);
             ++trials;
             const uint64_t slot = scale64(word64(r), Asz);
-            item = a.A[slot];
-            if (slot == 0 || a.A[slot - 1] != item) {
-                const double x = ((double)a.n * (double)w[item]) / W;
-                const double frac = x - floor(x);
-                if (coin53(r) > frac) continue;
-            }
+            // One 16-byte read per trial: the first slot of a run accepts with
+            // probability frac(N w / W), every other slot always.
+            const uint4 v = *reinterpret_cast<const uint4*>(&a.A[slot]);
+            item = (int32_t)v.z;
+            const double accept = __longlong_as_double(((long long)v.y << 32) | v.x);
+            if (coin53(r) > accept) continue;
             break;
         }
         emit(sa, g, item, csum);

@@ -43,6 +43,15 @@ struct alignas(16) Row {
     int32_t pad;
 };
 
+// One slot of the rejection array A (RejectionSampling.cu:26-56): the item and
+// the acceptance threshold of the slot: frac(N w / W) for the first slot of
+// the item's run, 1.0 (always accept) for the others.
+struct alignas(16) RejEntry {
+    double accept;
+    int32_t item;
+    int32_t pad;
+};
+
 struct Split {
     int32_t i, j;
     double spill;

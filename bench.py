@@ -67,6 +67,104 @@ def cpu_baseline(n_cpu, args):
             "single_thread_reference_psa_gitems_s": round(n_cpu / t_ref / 1e9, 4)}
 
 
+def time_builds(t, stream, steps, warmup):
+    """Mean ms of `steps` whole builds (HIP events on the build stream)."""
+    import torch
+    for _ in range(warmup):
+        t.build(stream)
+    torch.cuda.synchronize()
+    t.sync()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        t.build(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t.sync()
+    return e0.elapsed_time(e1) / steps
+
+
+def bench_learned(args, stream):
+    """Learned split index vs bisection on N=1e8 weights where 1% of the items
+    carry 90% of the mass (lognormal(10, 1) heavies rescaled, U(0,1) lights):
+    whole-build times at the same p, and bit identity of the two tables."""
+    import numpy as np
+    import atg
+    n = args.n
+    w = atg.generate_weights("heavy_mix", n, seed=42, param=0.9)
+    res = {"workload": f"N={n:.0e} heavy_mix (1% heavy, 90% mass), p={args.learned_p}, "
+                       f"{args.learned_segments} segments", "p": args.learned_p}
+    tabs = {}
+    for split in ("bisect", "learned"):
+        t = atg.Table(n, psa_plus=True, chains=args.chains, section_size=args.section_size,
+                      num_splits=args.learned_p, split=split, learned_segments=args.learned_segments,
+                      record_phases=True)
+        t.set_weights(w, stream)
+        ms = time_builds(t, stream, args.steps, args.warmup)
+        st = t.stats()
+        res[split] = {"build_ms": round(ms, 4), "split_ms": round(st["phase_ms"][3], 4),
+                      "gitems_s": round(n / (ms * 1e-3) / 1e9, 3)}
+        if split == "learned":
+            res[split]["learned_fallbacks"] = st["learned_fallbacks"]
+            res[split]["monotone"] = st["monotone"]
+        tabs[split] = t.table()
+        del t
+    res["speedup"] = round(res["bisect"]["build_ms"] / res["learned"]["build_ms"], 4)
+    res["split_speedup"] = round(res["bisect"]["split_ms"] / max(res["learned"]["split_ms"], 1e-9), 3)
+    a, b = tabs["bisect"], tabs["learned"]
+    res["bit_identical"] = bool(np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64)) and
+                                np.array_equal(a[1], b[1]))
+    log(f"[bench] learned vs plain: {res}")
+    return res
+
+
+def bench_batch(args, stream, name, ntables, lo, hi, dist, dtype, split, per_table, param=0.0):
+    """Batched build of many tables in one persistent launch (replayed hipGraph),
+    plus `per_table` samples from every table; per-table latencies from
+    device timestamps."""
+    import numpy as np
+    import torch
+    import atg
+    offsets, w = atg.generate_batch(dist, ntables, lo, hi, seed=42, param=param, dtype=dtype)
+    total = int(offsets[-1])
+    b = atg.Batch(offsets, dtype=dtype, split=split, waves=args.batch_waves)
+    b.set_weights(w, stream)
+    del w
+    out = torch.empty(ntables * per_table, dtype=torch.int32, device="cuda")
+    for _ in range(args.warmup):
+        b.build(stream)
+    torch.cuda.synchronize()
+    b.sync()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    K = args.steps
+    build_ms, sample_ms = [], []
+    for _ in range(K):
+        ev[0].record(stream)
+        b.build(stream)
+        ev[1].record(stream)
+        b.sample(per_table, seed=7, out=out, stream=stream)
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        build_ms.append(ev[0].elapsed_time(ev[1]))
+        sample_ms.append(ev[1].elapsed_time(ev[2]))
+    b.sync()
+    lat = b.latencies_ns() / 1e3
+    ms = statistics.mean(build_ms)
+    row_bytes = 16
+    w_bytes = 4 if dtype == "f32" else 8
+    res = {"workload": name, "tables": ntables, "items": total, "dtype": dtype, "split": split,
+           "build_ms": round(ms, 4), "gitems_s": round(total / (ms * 1e-3) / 1e9, 3),
+           "floor_hbm_frac": round(total * (2 * w_bytes + row_bytes) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "samples_per_table": per_table, "sample_ms": round(statistics.mean(sample_ms), 4),
+           "latency_us_p50": round(float(np.percentile(lat, 50)), 2),
+           "latency_us_p99": round(float(np.percentile(lat, 99)), 2),
+           "latency_us_max": round(float(lat.max()), 2)}
+    log(f"[bench] batched {name}: {res}")
+    del b, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +178,12 @@ def main():
     ap.add_argument("--split", choices=["bisect", "learned"], default="bisect")
     ap.add_argument("--learned-segments", type=int, default=4096)
     ap.add_argument("--sample-section", type=int, default=4096)
+    ap.add_argument("--learned-p", type=int, default=2621440, help="p of the learned-vs-plain comparison")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-waves", type=int, default=4)
+    ap.add_argument("--batch-learned", action="store_true",
+                    help="also the 16,384-table FP32 Zipf batch with the learned index (2.4e9 items)")
     ap.add_argument("--cpu-n", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sampling", action="store_true")
@@ -180,11 +284,16 @@ def main():
                               "samples": S, "section_rows": args.sample_section,
                               "sectioned_hbm_frac": round(sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log(f"[bench] sectioned sampling {S} in {ms:.3f} ms")
-        ev[0].record(stream)
-        t.rejection_build(stream)
-        ev[1].record(stream)
+        t.rejection_build(stream)  # first call allocates A
         torch.cuda.synchronize()
-        rb_ms = ev[0].elapsed_time(ev[1])
+        ts = []
+        for rep in range(3):
+            ev[0].record(stream)
+            t.rejection_build(stream)
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]))
+        rb_ms = statistics.median(ts)
         A = t.rejection_size()
         t.rejection_sample(S, seed=3, out=out, stream=stream)
         ts = []
@@ -195,12 +304,22 @@ def main():
             torch.cuda.synchronize()
             ts.append(ev[0].elapsed_time(ev[1]))
         rms = statistics.median(ts)
-        rbytes = 4 * S + (A / n) * S * 12
+        rbytes = 4 * S + (A / n) * S * 16  # one 16-byte slot per trial
         result["sampling"].update({"rejection_build_ms": round(rb_ms, 3), "rejection_A_over_N": round(A / n, 4),
                                    "rejection_gsamples_s": round(S / (rms * 1e-3) / 1e9, 3),
                                    "rejection_hbm_frac": round(rbytes / (rms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
         log(f"[bench] rejection build {rb_ms:.3f} ms, sampling {rms:.3f} ms")
         del out
+
+    if not args.no_learned:
+        result["learned"] = bench_learned(args, stream)
+    if not args.no_batch:
+        result["batched"] = [bench_batch(args, stream, "65,536 tables, sizes log-uniform [16, 2^16], Exp(1) FP64",
+                                         65536, 16, 1 << 16, "exponential", "f64", "bisect", 64)]
+        if args.batch_learned:
+            result["batched"].append(bench_batch(
+                args, stream, "16,384 tables, sizes log-uniform [2^10, 2^20], FP32 rank-Zipf(1.2) shuffled, learned",
+                16384, 1 << 10, 1 << 20, "zipf_shuffled", "f32", "learned", 256, param=1.2))
 
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline on N={args.cpu_n}")

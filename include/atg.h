@@ -59,7 +59,7 @@ typedef struct {
     int32_t split_method;    /* atg_split_method */
     int32_t learned_segments;/* knots of the learned split index (default 4096) */
     int32_t record_phases;   /* 1: hipEvents around sum/partition/prefixsum/split/pack */
-    int32_t reserved;        /* 0 */
+    int32_t batch_waves;     /* batched build: waves per table workgroup (1..16); 0 = 4 */
 } atg_build_opts;
 
 typedef struct {
@@ -179,6 +179,12 @@ typedef enum {
 
 /* Host FP64 weights from a Philox4x32-10 stream (key = seed). */
 atg_status atg_generate_weights(int32_t dist, int64_t n, uint64_t seed, double param, double* out);
+/* Host weights of a batch of tables: sizes log-uniform over [lo, hi] into
+ * offsets[0..ntables]; with out != NULL also the weights of every table
+ * (table t from seed + t + 1), concatenated, as FP64 or FP32 (out_dtype).
+ * Workload synthesis; no counterpart in the reference. */
+atg_status atg_generate_batch(int32_t dist, int64_t ntables, int64_t lo, int64_t hi, uint64_t seed,
+                              double param, int64_t* offsets, void* out, int32_t out_dtype);
 
 /* ---- batched build: many independent tables ------------------------------ */
 

@@ -510,15 +510,27 @@ static inline int32_t decide(double rw, int32_t alias, int64_t row, double u, do
     return (int32_t)row;
 }
 
-extern "C" void oracle_sample_basic(const double* row_w, const int32_t* row_alias, int64_t n, double W_N,
-                                   uint64_t seed, uint64_t first, uint64_t count, int32_t* out) {
+static void sample_rows(const double* row_w, const int32_t* row_alias, int64_t n, double W_N, uint64_t seed,
+                        uint32_t purpose, uint64_t first, uint64_t count, int32_t* out) {
 #pragma omp parallel for schedule(static)
     for (int64_t t = 0; t < (int64_t)count; ++t) {
         uint32_t x[4];
-        philox_u64(first + (uint64_t)t, ORACLE_PURPOSE_BASIC, 0, seed, x);
+        philox_u64(first + (uint64_t)t, purpose, 0, seed, x);
         const int64_t row = (int64_t)mulhi64(word64(x), (uint64_t)n);
         out[t] = decide(row_w[row], row_alias[row], row, coin53(x), W_N, n);
     }
+}
+
+extern "C" void oracle_sample_basic(const double* row_w, const int32_t* row_alias, int64_t n, double W_N,
+                                   uint64_t seed, uint64_t first, uint64_t count, int32_t* out) {
+    sample_rows(row_w, row_alias, n, W_N, seed, ORACLE_PURPOSE_BASIC, first, count, out);
+}
+
+/* Samples of one table of a batch: the Philox counters are the global output
+ * positions first .. first + count - 1 under the batch purpose. */
+extern "C" void oracle_sample_batch_table(const double* row_w, const int32_t* row_alias, int64_t n, double W_N,
+                                          uint64_t seed, uint64_t first, uint64_t count, int32_t* out) {
+    sample_rows(row_w, row_alias, n, W_N, seed, ORACLE_PURPOSE_BATCH, first, count, out);
 }
 
 /* Binomial draw for the sectioned sample counts.  The reference halves the

@@ -45,6 +45,7 @@ extern "C" {
 #define ORACLE_PURPOSE_SECTIONED 0x0000B002u
 #define ORACLE_PURPOSE_SECTION_COUNTS 0x0000B003u
 #define ORACLE_PURPOSE_REJECTION 0x0000B004u
+#define ORACLE_PURPOSE_BATCH 0x0000B005u
 #define ORACLE_REJECTION_MAX_TRIALS 256
 
 typedef struct {
@@ -107,6 +108,9 @@ void oracle_verify_table(const double* w, int64_t n, double W, double W_N,
 void oracle_sample_basic(const double* row_w, const int32_t* row_alias, int64_t n,
                          double W_N, uint64_t seed, uint64_t first, uint64_t count,
                          int32_t* out);
+void oracle_sample_batch_table(const double* row_w, const int32_t* row_alias, int64_t n,
+                               double W_N, uint64_t seed, uint64_t first, uint64_t count,
+                               int32_t* out);
 uint64_t oracle_binomial(uint64_t n, double q, uint64_t seed, uint64_t node);
 void oracle_section_counts(int64_t n, int64_t section_rows, uint64_t total, uint64_t seed,
                            uint64_t* counts /* ceil(n/section_rows) */);

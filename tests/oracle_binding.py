@@ -51,6 +51,7 @@ L.oracle_split_one.restype = C.c_int
 L.oracle_split_one.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_double, P, C.c_int64, P, P, C.c_int64, P, P, P]
 L.oracle_verify_table.argtypes = [P, C.c_int64, C.c_double, C.c_double, P, P, P, P, P, P]
 L.oracle_sample_basic.argtypes = [P, P, C.c_int64, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, P]
+L.oracle_sample_batch_table.argtypes = [P, P, C.c_int64, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, P]
 L.oracle_binomial.restype = C.c_uint64
 L.oracle_binomial.argtypes = [C.c_uint64, C.c_double, C.c_uint64, C.c_uint64]
 L.oracle_section_counts.argtypes = [C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, P]
@@ -146,6 +147,14 @@ def split_one(k, p, nToSplit, wToSplit, Lp, Hp, Hw):
 def sample_basic(row_w, row_alias, W_N, seed, count, first=0):
     out = np.empty(int(count), np.int32)
     L.oracle_sample_basic(ptr(row_w), ptr(row_alias), len(row_w), W_N, int(seed), int(first), int(count), ptr(out))
+    return out
+
+
+def sample_batch_table(row_w, row_alias, W_N, seed, count, first):
+    """Samples first .. first + count - 1 of a batch, drawn from one table."""
+    out = np.empty(int(count), np.int32)
+    L.oracle_sample_batch_table(ptr(row_w), ptr(row_alias), len(row_w), W_N, int(seed), int(first), int(count),
+                                ptr(out))
     return out
 
 
