@@ -66,6 +66,8 @@ struct atg_table_s {
     double* d_section_sums = nullptr;
     double* d_tree = nullptr;
     uint64_t* d_slots = nullptr;
+    double* d_stage_w = nullptr;
+    int32_t* d_stage_i = nullptr;
     int32_t *d_Li = nullptr, *d_Hi = nullptr;
     double *d_Lw = nullptr, *d_Lp = nullptr, *d_Hw = nullptr, *d_Hp = nullptr;
     Row* d_rows = nullptr;
@@ -99,6 +101,8 @@ struct atg_table_s {
         a.tree_scratch = d_tree;
         a.tree_scratch_half = tree_half;
         a.slots = d_slots;
+        a.stage_w = d_stage_w;
+        a.stage_i = d_stage_i;
         a.Li = d_Li; a.Lw = d_Lw; a.Lp = d_Lp;
         a.Hi = d_Hi; a.Hw = d_Hw; a.Hp = d_Hp;
         a.rows = d_rows;
@@ -137,7 +141,8 @@ int32_t atg_device_count(void) {
 
 static void free_table(atg_table* t) {
     if (!t) return;
-    void* ptrs[] = {t->d_weights_owned, t->d_state, t->d_section_sums, t->d_tree, t->d_slots, t->d_Li, t->d_Hi,
+    void* ptrs[] = {t->d_weights_owned, t->d_state, t->d_section_sums, t->d_tree, t->d_slots, t->d_stage_w,
+                    t->d_stage_i, t->d_Li, t->d_Hi,
                     t->d_Lw, t->d_Lp, t->d_Hw, t->d_Hp, t->d_rows, t->d_tw, t->d_ta, t->d_splits, t->d_knots,
                     t->d_checksum, t->d_A, t->d_rej_tot, t->d_rej_off};
     for (void* p : ptrs)
@@ -200,6 +205,8 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     ALLOC(t->d_Lp, n + 2);
     ALLOC(t->d_Hw, n + 2);
     ALLOC(t->d_Hp, n + 2);
+    ALLOC(t->d_stage_w, n);
+    ALLOC(t->d_stage_i, n);
     if (layout == ATG_AOS) {
         ALLOC(t->d_rows, n);
     } else {

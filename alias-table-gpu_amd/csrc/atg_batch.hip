@@ -10,10 +10,9 @@
 // the same association, so a table built here is bit-identical to the same
 // weights built alone with the same options:
 //   W       : section sums by the waves, then the 64-ary tree level by level;
-//   PSA+    : wave w takes sections w, w + nw, ... in increasing order; the
-//             dyadic section prefixes are published and awaited exactly as in
-//             the single-table kernel (a section waits only for lower
-//             sections, which are already taken by resident waves);
+//   PSA+    : wave w takes sections w, w + nw, ...: pass 1 (partition,
+//             chains, staged leftovers, dyadic block sums), a barrier, then
+//             pass 2 (section prefixes, leftover lists, certificate);
 //   split   : bisection or the learned index per k;
 //   pack    : one chain per k.
 // Device timestamps (the 100 MHz wall clock) at the start and end of every
@@ -62,13 +61,34 @@ struct BatchSampleArgs {
     int32_t* out;             // ntables * per_table
 };
 
-__global__ void batch_reset(BatchQueue* q) {
-    if (threadIdx.x == 0) q->next = 0u;
+// Before every build (a separate launch, so that none of these plain stores
+// can land after the build kernel's atomics): the queue, every table's state
+// and its dyadic-block slots (sums not ready, arrival counters zero).
+__global__ __launch_bounds__(256) void batch_reset(const BuildArgs* tables, int64_t ntables, BatchQueue* q) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) q->next = 0u;
+    for (int64_t t = blockIdx.x; t < ntables; t += gridDim.x) {
+        const BuildArgs& a = tables[t];
+        if (threadIdx.x == 0) {
+            DevState* st = a.st;
+            st->W = 0.0;
+            st->W_N = 0.0;
+            st->error = 0;
+            st->degenerate = 0;
+            st->numL = st->numH = st->p = st->nToSplit = 0;
+            st->wToSplit = 0.0;
+            st->cert_max_dL = 0ull;                   // +0.0
+            st->cert_min_dH = 0x7FF0000000000000ull;  // +inf
+            st->split_fallbacks = 0;
+            st->learned_fallbacks = 0;
+        }
+        const int64_t nvals = 3 * a.tree_total, ncnt = (a.tree_total + 1) / 2;
+        for (int64_t i = threadIdx.x; i < nvals + ncnt; i += blockDim.x) a.slots[i] = (i < nvals) ? ATG_NOT_READY : 0ull;
+    }
 }
 
 // W of one table: section sums (lane l adds items l, l + 64, ...), then the
 // 64-ary tree over the section sums, 64 values per wave and level, zero-padded
-// (the association of sum_sections).  Also resets the section-prefix slots.
+// (the association of sum_sections).
 template <typename T>
 __device__ __forceinline__ void wg_weight_sum(const BuildArgs& a, int tid, int nthreads, uint32_t* s_err) {
     const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
@@ -77,8 +97,6 @@ __device__ __forceinline__ void wg_weight_sum(const BuildArgs& a, int tid, int n
         const double sum = section_sum<T>(w, min(a.n, s * a.Ls), min(a.n, (s + 1) * a.Ls), lane);
         if (lane == 0) a.section_sums[s] = sum;
     }
-    const int64_t nvals = 3 * a.tree_total, ncnt = (a.tree_total + 1) / 2;
-    for (int64_t i = tid; i < nvals + ncnt; i += nthreads) a.slots[i] = (i < nvals) ? ATG_NOT_READY : 0ull;
     __syncthreads();
     double* bufs[2] = {a.tree_scratch, a.tree_scratch + a.tree_scratch_half};
     const double* src = a.section_sums;
@@ -106,14 +124,7 @@ __device__ __forceinline__ void wg_weight_sum(const BuildArgs& a, int tid, int n
         if (a.Ls > a.Ls_max) err |= ERR_SECTION_TOO_LONG;
         st->W = W;
         st->W_N = W / (double)a.n;
-        st->error = err;
-        st->ticket = 0;
-        st->degenerate = 0;
-        st->cert_max_dL = 0ull;
-        st->cert_min_dH = 0x7FF0000000000000ull;
-        st->split_fallbacks = 0;
-        st->learned_fallbacks = 0;
-        st->sum_done = 0;
+        if (err) atomicOr(&st->error, err);
         *s_err = err;
     }
     __syncthreads();
@@ -127,9 +138,11 @@ __device__ __forceinline__ void wg_build_table(const BuildArgs& a, double* lds_r
     wg_weight_sum<T>(a, tid, nthreads, s_word);
     if (*s_word) return;
 
-    // PSA+ / partition with the fused prefix sums.
+    // PSA+ / partition, then the leftover lists with their prefix sums.
     double* mine = lds_raw + (int64_t)wave * wave_lds_doubles;
-    for (int64_t s = wave; s < a.G; s += nw) psa_section<T, true>(a, s, mine, lane);
+    for (int64_t s = wave; s < a.G; s += nw) psa_local<T, true>(a, s, mine, lane);
+    __syncthreads();
+    for (int64_t s = wave; s < a.G; s += nw) place_section(a, s, lane);
     __syncthreads();
     if (tid == 0) *s_word = st->error | (st->degenerate ? 0x80000000u : 0u);
     __syncthreads();
@@ -286,7 +299,8 @@ hipError_t launch_batch(const atg_batch* b, hipStream_t s) {
     k.ticks = b->d_ticks;
     k.wave_lds_doubles = (int64_t)(psa_lds_bytes(b->Ls_max) / sizeof(double));
     k.learned = b->opts.split_method == ATG_SPLIT_LEARNED;
-    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, b->d_queue);
+    const unsigned rg = (unsigned)std::min<int64_t>(b->ntables, 4096);
+    hipLaunchKernelGGL(batch_reset, dim3(rg), dim3(256), 0, s, (const BuildArgs*)b->d_args, b->ntables, b->d_queue);
     hipLaunchKernelGGL(batch_build<T>, dim3(b->grid), dim3(64 * b->nwaves), b->lds_bytes, s, k);
     return hipGetLastError();
 }
@@ -349,11 +363,11 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         z.half = (z.G + 63) / 64 + 1;
         z.tree_total = block_tree_total(z.G);
         od[t] = nd;
-        nd += align_up(z.G, 32) + align_up(2 * z.half, 32) + 4 * align_up(n + 2, 32);
+        nd += align_up(z.G, 32) + align_up(2 * z.half, 32) + 5 * align_up(n + 2, 32);
         os[t] = ns;
         ns += align_up(3 * z.tree_total + (z.tree_total + 1) / 2 + 1, 32);
         oi[t] = ni;
-        ni += 2 * align_up(n + 2, 64) + align_up(o.learned_segments + 2, 64);
+        ni += 3 * align_up(n + 2, 64) + align_up(o.learned_segments + 2, 64);
         osp[t] = nsp;
         nsp += align_up(z.p_req + 1, 16);
     }
@@ -396,11 +410,11 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         a.tree_scratch_half = z.half;
         d += align_up(2 * z.half, 32);
         const int64_t cap = align_up(n + 2, 32);
-        a.Lw = d; a.Lp = d + cap; a.Hw = d + 2 * cap; a.Hp = d + 3 * cap;
+        a.Lw = d; a.Lp = d + cap; a.Hw = d + 2 * cap; a.Hp = d + 3 * cap; a.stage_w = d + 4 * cap;
         a.slots = b->d_slots + os[t];
         int32_t* ip = b->d_ipool + oi[t];
         const int64_t icap = align_up(n + 2, 64);
-        a.Li = ip; a.Hi = ip + icap; a.knot_j = ip + 2 * icap;
+        a.Li = ip; a.Hi = ip + icap; a.stage_i = ip + 2 * icap; a.knot_j = ip + 3 * icap;
         a.rows = b->d_rows + offsets[t];
         a.tw = nullptr;
         a.ta = nullptr;

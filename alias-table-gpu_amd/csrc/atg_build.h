@@ -21,7 +21,9 @@ struct BuildArgs {
     double* section_sums;     // G
     double* tree_scratch;     // 2 * tree_scratch_half
     int64_t tree_scratch_half;
-    uint64_t* slots;          // 6 * G look-back slots: aggC aggL aggH inclC inclL inclH
+    uint64_t* slots;          // dyadic block sums (counts, light, heavy) + arrival counters
+    double* stage_w;          // n: leftovers staged by partition pass 1 (weights)
+    int32_t* stage_i;         // n: leftovers staged by partition pass 1 (items)
     int32_t* Li; double* Lw; double* Lp;  // leftover lights (capacity n + 2)
     int32_t* Hi; double* Hw; double* Hp;  // leftover heavies (capacity n + 2)
     Row* rows;                // AoS table or null

@@ -5,14 +5,16 @@
 //   1. sum_sections : per-section FP64 sums (64 sections per block), weight
 //                     validation, the first level of the canonical 64-ary tree;
 //                     the last block to finish completes the tree -> W, W/N.
-//   2. psa_wave     : one wave per section, sections taken in ticket order:
-//                     all loads issued at once, stable light/heavy partition
-//                     into LDS, PSA+ greedy chains writing finished rows,
-//                     leftover maps, the canonical local scan of the leftover
-//                     weights, the section prefix from dyadic block sums that
-//                     the sections complete cooperatively (O(log G) depth, the
-//                     association the oracle defines), leftover lists + prefix
-//                     sums, the split certificate and the sentinels.
+//   2. partition_local : one wave per section: all loads issued at once,
+//                     stable light/heavy partition into LDS, PSA+ greedy
+//                     chains writing finished rows, leftover order, the
+//                     canonical local-scan aggregates published into the
+//                     dyadic block tree of section sums (completed by the
+//                     second child to arrive), leftovers staged in place.
+//      partition_place : per section, P(s) from O(log G) completed blocks
+//                     (the association the oracle defines), leftover lists
+//                     with prefix sums, the split certificate, the sentinels.
+//                     No section ever waits for another to finish its work.
 //   3. degenerate_fill (only when one leftover class is empty), split, pack.
 #include <algorithm>
 
@@ -84,7 +86,6 @@ __global__ __launch_bounds__(1024) void sum_sections(BuildArgs a) {
         st->W = W;
         st->W_N = W / (double)a.n;
         st->error = err;
-        st->ticket = 0;
         st->degenerate = 0;
         st->cert_max_dL = 0ull;                    // +0.0
         st->cert_min_dH = 0x7FF0000000000000ull;   // +inf
@@ -95,22 +96,24 @@ __global__ __launch_bounds__(1024) void sum_sections(BuildArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// 2. PSA+ / plain partition: one wave per section, sections taken in ticket
-//    order.  The section prefixes of the leftover sums use the dyadic-block
-//    association of the oracle: each section publishes its aggregate as the
-//    level-0 block, the child that completes a pair second publishes the
-//    parent, and P(s) folds the blocks of s's binary digits from high to low.
+// 2. PSA+ / plain partition in two passes.  The section prefixes of the
+//    leftover sums use the dyadic-block association of the oracle: each
+//    section publishes its aggregate as the level-0 block, the child that
+//    completes a pair second publishes the parent (pass 1), and P(s) folds
+//    the blocks of s's binary digits from high to low (pass 2).
 template <typename T, bool AOS>
-__global__ __launch_bounds__(64) void psa_wave(BuildArgs a) {
+__global__ __launch_bounds__(64) void partition_local(BuildArgs a) {
     extern __shared__ double lds_raw[];
-    DevState* st = a.st;
-    if (st->error) return;
-    const int lane = threadIdx.x;
-    uint32_t sid = 0;
-    if (lane == 0) sid = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int64_t s = (int64_t)__shfl(sid, 0, 64);
+    if (a.st->error) return;
+    psa_local<T, AOS>(a, (int64_t)blockIdx.x, lds_raw, (int)threadIdx.x);
+}
+
+// Four sections per block, one per wave.
+__global__ __launch_bounds__(256) void partition_place(BuildArgs a) {
+    if (a.st->error) return;
+    const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (s >= a.G) return;
-    psa_section<T, AOS>(a, s, lds_raw, lane);
+    place_section(a, s, (int)(threadIdx.x & 63));
 }
 
 // ---------------------------------------------------------------------------
@@ -279,12 +282,13 @@ hipError_t launch_partition(const BuildArgs& a, hipStream_t s) {
     const size_t lds = psa_lds_bytes(a.Ls_max);
     const dim3 g((unsigned)a.G), b(64);
     if (a.dtype_f32) {
-        if (a.rows) hipLaunchKernelGGL((psa_wave<float, true>), g, b, lds, s, a);
-        else hipLaunchKernelGGL((psa_wave<float, false>), g, b, lds, s, a);
+        if (a.rows) hipLaunchKernelGGL((partition_local<float, true>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((partition_local<float, false>), g, b, lds, s, a);
     } else {
-        if (a.rows) hipLaunchKernelGGL((psa_wave<double, true>), g, b, lds, s, a);
-        else hipLaunchKernelGGL((psa_wave<double, false>), g, b, lds, s, a);
+        if (a.rows) hipLaunchKernelGGL((partition_local<double, true>), g, b, lds, s, a);
+        else hipLaunchKernelGGL((partition_local<double, false>), g, b, lds, s, a);
     }
+    hipLaunchKernelGGL(partition_place, dim3(blocks_for(a.G, 4)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -326,8 +330,8 @@ hipError_t launch_split_rows(const Row* rows, double* w, int32_t* al, int64_t n,
 }
 
 hipError_t set_partition_lds_limit(size_t bytes) {
-    const void* fns[] = {(const void*)psa_wave<double, true>, (const void*)psa_wave<double, false>,
-                         (const void*)psa_wave<float, true>, (const void*)psa_wave<float, false>};
+    const void* fns[] = {(const void*)partition_local<double, true>, (const void*)partition_local<double, false>,
+                         (const void*)partition_local<float, true>, (const void*)partition_local<float, false>};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;

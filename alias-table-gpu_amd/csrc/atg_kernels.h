@@ -152,11 +152,13 @@ __device__ Prefix block_prefix(const BuildArgs& a, int64_t s, int lane, bool& ti
     return P;
 }
 
-// PSA+ (or the plain partition) of section s by one wave: stable partition
-// into LDS, greedy chains, leftover lists with their global prefix sums,
-// the split certificate; the last section also writes the totals.
+// Partition pass 1, section s by one wave: stable light/heavy partition into
+// LDS, PSA+ greedy chains writing finished rows, the leftover order, the
+// canonical local-scan aggregates (published as the level-0 block of the
+// dyadic section tree) and the leftovers staged in rank order at the section's
+// own range (lights, then heavies).  Nothing here waits for another section.
 template <typename T, bool AOS>
-__device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, double* lds_raw, int lane) {
+__device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double* lds_raw, int lane) {
     DevState* st = a.st;
     const T* w = (const T*)a.weights;
     const int64_t start = min(a.n, s * a.Ls);
@@ -277,17 +279,43 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, doubl
         aggH = aggH + readlane_d(wave_scan(v), 63);
     }
     const uint64_t aggC = (uint64_t)(uint32_t)mL | ((uint64_t)(uint32_t)mH << 32);
+    // Stage the leftovers (weight, item) in rank order: lights at
+    // [start, start + mL), heavies at [start + mL, start + mL + mH).
+    for (int r0 = 0; r0 < mL + mH; r0 += 64) {
+        const int r = r0 + lane;
+        if (r < mL + mH) {
+            const bool heavy = r >= mL;
+            const int q = heavy ? r - mL : r;
+            const int pos = identity ? q : (int)L.map[r];
+            a.stage_w[start + r] = heavy ? L.Hw(pos) : L.Lw(pos);
+            a.stage_i[start + r] = (int32_t)(start + (heavy ? L.Hi(pos) : L.Li(pos)));
+        }
+    }
     bool timeout = false;
     if (lane == 0) block_publish(a, s, aggC, aggL, aggH, timeout);
-    const Prefix P = block_prefix(a, s, lane, timeout);
+    if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
+}
 
-    // Leftover lists with global prefixes P + local, and the split certificate.
+// Partition pass 2, section s by one wave: the section prefix P(s) from the
+// completed dyadic blocks, then the staged leftovers with prefixes
+// P(s) + local (the canonical chunked scan recomputed in the same order),
+// the split certificate, and for the last section the totals, sentinels and
+// split inputs.
+__device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int lane) {
+    DevState* st = a.st;
+    const int64_t start = min(a.n, s * a.Ls);
+    bool timeout = false;
+    uint64_t c0, l0, h0;
+    block_load(a, s, c0, l0, h0, timeout);  // this section's own counts
+    const int mL = (int)(uint32_t)c0, mH = (int)(c0 >> 32);
+    const Prefix P = block_prefix(a, s, lane, timeout);
     const int64_t goffL = (int64_t)(uint32_t)P.c, goffH = (int64_t)(P.c >> 32);
     double ext_l = 0.0, ext_h = __builtin_inf();
     for (int hv = 0; hv < 2; ++hv) {
         const bool heavy = hv == 1;
         const int m = heavy ? mH : mL;
-        const double Pv = heavy ? P.h : P.l, agg = heavy ? aggH : aggL;
+        const int64_t src = start + (heavy ? mL : 0);
+        const double Pv = heavy ? P.h : P.l;
         int32_t* oi = heavy ? a.Hi : a.Li;
         double* ow = heavy ? a.Hw : a.Lw;
         double* op = heavy ? a.Hp : a.Lp;
@@ -296,21 +324,19 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, doubl
         for (int r0 = 0; r0 < m; r0 += 64) {
             const int r = r0 + lane;
             double v = 0.0;
-            int item = 0;
+            int32_t item = 0;
             if (r < m) {
-                const int pos = identity ? r : (int)L.map[(heavy ? mL : 0) + r];
-                v = heavy ? L.Hw(pos) : L.Lw(pos);
-                item = heavy ? L.Hi(pos) : L.Li(pos);
+                v = a.stage_w[src + r];
+                item = a.stage_i[src + r];
             }
             const double inc = wave_scan(v);
             const double local = carry + wave_shr1(inc);
             const double pref = Pv + local;
             const double carry_next = carry + readlane_d(inc, 63);
             double succ = __shfl_down(pref, 1, 64);
-            if (lane == 63) succ = Pv + carry_next;
-            if (r == m - 1) succ = Pv + agg;
+            if (lane == 63 || r == m - 1) succ = Pv + carry_next;
             if (r < m) {
-                oi[goff + r] = (int32_t)(start + item);
+                oi[goff + r] = item;
                 ow[goff + r] = v;
                 op[goff + r] = pref;
                 const double d = succ - pref;
@@ -348,6 +374,7 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, doubl
     }
     if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
 }
+
 
 // ---------------------------------------------------------------------------
 // Pack helpers.

@@ -20,7 +20,7 @@ struct DevState {
     double W;
     double W_N;
     uint32_t error;
-    uint32_t ticket;          // dynamic section ids of the PSA pass
+    uint32_t pad2;
     uint32_t degenerate;
     uint32_t pad0;
     int64_t numL, numH;       // incl. sentinels
