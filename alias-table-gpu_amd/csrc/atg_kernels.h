@@ -94,7 +94,7 @@ __device__ __forceinline__ void block_load(const BuildArgs& a, int64_t idx, uint
         l = slot_load(V + a.tree_total + idx);
         h = slot_load(V + 2 * a.tree_total + idx);
         if (c != ATG_NOT_READY && l != ATG_NOT_READY && h != ATG_NOT_READY) return;
-        if (++spins > (1u << 22)) { timeout = true; c = 0; l = d2u(0.0); h = d2u(0.0); return; }
+        if (++spins > (1u << 18)) { timeout = true; c = 0; l = d2u(0.0); h = d2u(0.0); return; }
         __builtin_amdgcn_s_sleep(1);
     }
 }
