@@ -11,8 +11,9 @@
 // weights built alone with the same options:
 //   W       : section sums by the waves, then the 64-ary tree level by level;
 //   PSA+    : wave w takes sections w, w + nw, ...: pass 1 (partition,
-//             chains, staged leftovers, dyadic block sums), a barrier, then
-//             pass 2 (section prefixes, leftover lists, certificate);
+//             chains, staged leftovers, level-0 block sums), a barrier, the
+//             upper block sums level by level, then pass 2 (section
+//             prefixes, leftover lists, certificate);
 //   split   : bisection or the learned index per k;
 //   pack    : one chain per k.
 // Device timestamps (the 100 MHz wall clock) at the start and end of every
@@ -31,8 +32,9 @@
 namespace atg {
 
 struct BatchQueue {
-    uint32_t next;
-    uint32_t pad[63];
+    uint32_t next;   // next table ticket
+    uint32_t done;   // workgroups that found the queue empty
+    uint32_t pad[62];
 };
 
 // Per-table build state on its own 256-byte line pair, so that no cache line
@@ -61,36 +63,21 @@ struct BatchSampleArgs {
     int32_t* out;             // ntables * per_table
 };
 
-// Before every build (a separate launch, so that none of these plain stores
-// can land after the build kernel's atomics): the queue, every table's state
-// and its dyadic-block slots (sums not ready, arrival counters zero).
-__global__ __launch_bounds__(256) void batch_reset(const BuildArgs* tables, int64_t ntables, BatchQueue* q) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) q->next = 0u;
-    for (int64_t t = blockIdx.x; t < ntables; t += gridDim.x) {
-        const BuildArgs& a = tables[t];
-        if (threadIdx.x == 0) {
-            DevState* st = a.st;
-            st->W = 0.0;
-            st->W_N = 0.0;
-            st->error = 0;
-            st->degenerate = 0;
-            st->numL = st->numH = st->p = st->nToSplit = 0;
-            st->wToSplit = 0.0;
-            st->cert_max_dL = 0ull;                   // +0.0
-            st->cert_min_dH = 0x7FF0000000000000ull;  // +inf
-            st->split_fallbacks = 0;
-            st->learned_fallbacks = 0;
-        }
-        const int64_t nvals = 3 * a.tree_total, ncnt = (a.tree_total + 1) / 2;
-        for (int64_t i = threadIdx.x; i < nvals + ncnt; i += blockDim.x) a.slots[i] = (i < nvals) ? ATG_NOT_READY : 0ull;
-    }
-}
+// Workgroup-shared scratch of one table build.
+struct TableLDS {
+    BuildArgs a;
+    uint32_t idx;
+    uint32_t flags;
+    uint32_t split_fallbacks;
+    uint32_t learned_fallbacks;
+    uint64_t cert_l, cert_h;  // bit patterns (non-negative doubles order like integers)
+};
 
 // W of one table: section sums (lane l adds items l, l + 64, ...), then the
 // 64-ary tree over the section sums, 64 values per wave and level, zero-padded
-// (the association of sum_sections).
+// (the association of sum_sections).  Returns the error bits.
 template <typename T>
-__device__ __forceinline__ void wg_weight_sum(const BuildArgs& a, int tid, int nthreads, uint32_t* s_err) {
+__device__ __forceinline__ uint32_t wg_weight_sum(const BuildArgs& a, int tid, int nthreads, TableLDS& sh) {
     const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
     const T* w = (const T*)a.weights;
     for (int64_t s = wave; s < a.G; s += nw) {
@@ -124,31 +111,48 @@ __device__ __forceinline__ void wg_weight_sum(const BuildArgs& a, int tid, int n
         if (a.Ls > a.Ls_max) err |= ERR_SECTION_TOO_LONG;
         st->W = W;
         st->W_N = W / (double)a.n;
-        if (err) atomicOr(&st->error, err);
-        *s_err = err;
+        st->error = err;
+        st->degenerate = 0;
+        st->p = 0;
+        sh.flags = err;
+        sh.split_fallbacks = 0;
+        sh.learned_fallbacks = 0;
+        sh.cert_l = 0ull;                   // +0.0
+        sh.cert_h = 0x7FF0000000000000ull;  // +inf
     }
     __syncthreads();
+    return sh.flags;
 }
 
+// One table by one workgroup.  Communication between the waves goes through
+// plain global memory ordered by workgroup barriers and through LDS; there is
+// no waiting on another wave's progress and no global atomic.
 template <typename T>
 __device__ __forceinline__ void wg_build_table(const BuildArgs& a, double* lds_raw, int64_t wave_lds_doubles,
-                                               bool learned, int tid, int nthreads, uint32_t* s_word) {
+                                               bool learned, int tid, int nthreads, TableLDS& sh) {
     const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
     DevState* st = a.st;
-    wg_weight_sum<T>(a, tid, nthreads, s_word);
-    if (*s_word) return;
+    if (wg_weight_sum<T>(a, tid, nthreads, sh)) return;
 
-    // PSA+ / partition, then the leftover lists with their prefix sums.
+    // Partition pass 1 per section, the dyadic block sums, pass 2 per section.
     double* mine = lds_raw + (int64_t)wave * wave_lds_doubles;
-    for (int64_t s = wave; s < a.G; s += nw) psa_local<T, true>(a, s, mine, lane);
+    for (int64_t s = wave; s < a.G; s += nw) psa_local<T, true, false>(a, s, mine, lane);
     __syncthreads();
-    for (int64_t s = wave; s < a.G; s += nw) place_section(a, s, lane);
+    wg_block_tree(a, tid, nthreads);
+    double ext_l = 0.0, ext_h = __builtin_inf();
+    for (int64_t s = wave; s < a.G; s += nw) place_section<true>(a, s, lane, ext_l, ext_h);
+    if (lane == 0) {
+        atomicMax((unsigned long long*)&sh.cert_l, (unsigned long long)d2u(ext_l));
+        atomicMin((unsigned long long*)&sh.cert_h, (unsigned long long)d2u(ext_h));
+    }
     __syncthreads();
-    if (tid == 0) *s_word = st->error | (st->degenerate ? 0x80000000u : 0u);
+    if (tid == 0) {
+        st->cert_max_dL = sh.cert_l;
+        st->cert_min_dH = sh.cert_h;
+        sh.flags = st->degenerate;
+    }
     __syncthreads();
-    const uint32_t flags = *s_word;
-    if (flags & 0x7FFFFFFFu) return;
-    if (flags & 0x80000000u) {
+    if (sh.flags) {
         // One leftover class is empty: every leftover row carries its own mass.
         const int64_t nLr = st->numL - 1, nHr = st->numH - 1;
         for (int64_t t = tid; t < nLr + nHr; t += nthreads) {
@@ -156,16 +160,25 @@ __device__ __forceinline__ void wg_build_table(const BuildArgs& a, double* lds_r
             const double w = (t < nLr) ? a.Lw[t] : a.Hw[t - nLr];
             store_row<true>(a, item, w, item);
         }
+        if (tid == 0) {
+            st->split_fallbacks = 0;
+            st->learned_fallbacks = 0;
+        }
         return;
     }
 
     // Split.
     const int64_t p = st->p, numL = st->numL, numH = st->numH;
+    uint32_t sfb = 0, lfb = 0;
     if (learned) {
         const int64_t K = min<int64_t>(a.learned_segments, p);
-        for (int64_t m = tid; m <= K; m += nthreads) knot_split(a, m);
+        for (int64_t m = tid; m <= K; m += nthreads) sfb += knot_split(a, m);
         __syncthreads();
-        for (int64_t k = 1 + tid; k < p; k += nthreads) learned_split(a, k);
+        for (int64_t k = 1 + tid; k < p; k += nthreads) {
+            const uint32_t f = learned_split(a, k);
+            sfb += f & 1u;
+            lfb += (f >> 1) & 1u;
+        }
     } else {
         if (tid == 0) {
             a.splits[0] = Split{0, 0, 0.0};
@@ -173,12 +186,17 @@ __device__ __forceinline__ void wg_build_table(const BuildArgs& a, double* lds_r
         }
         for (int64_t k = 1 + tid; k < p; k += nthreads) {
             Split sp;
-            if (!bisect_split(k, p, st->nToSplit, st->wToSplit, a.Lp, numL, a.Hp, a.Hw, numH, sp))
-                atomicAdd((unsigned long long*)&st->split_fallbacks, 1ull);
+            if (!bisect_split(k, p, st->nToSplit, st->wToSplit, a.Lp, numL, a.Hp, a.Hw, numH, sp)) ++sfb;
             a.splits[k] = sp;
         }
     }
+    if (sfb) atomicAdd(&sh.split_fallbacks, sfb);
+    if (lfb) atomicAdd(&sh.learned_fallbacks, lfb);
     __syncthreads();
+    if (tid == 0) {
+        st->split_fallbacks = sh.split_fallbacks;
+        st->learned_fallbacks = sh.learned_fallbacks;
+    }
 
     // Pack: one chain per section of the split.
     const double W_N = st->W_N;
@@ -186,26 +204,36 @@ __device__ __forceinline__ void wg_build_table(const BuildArgs& a, double* lds_r
     for (int64_t k = 1 + tid; k <= p; k += nthreads) pack_chain<true>(a, l, a.splits[k - 1], a.splits[k], W_N);
 }
 
+// Persistent: every workgroup takes tables until the queue is empty; the last
+// workgroup to find it empty resets the queue for the next launch (so a
+// replayed graph needs no reset node).
 template <typename T>
 __global__ __launch_bounds__(1024) void batch_build(BatchBuildArgs b) {
     extern __shared__ double lds_raw[];
-    __shared__ BuildArgs sa;
-    __shared__ uint32_t s_idx, s_word;
+    __shared__ TableLDS sh;
     const int tid = threadIdx.x, nthreads = blockDim.x;
     while (true) {
-        if (tid == 0) s_idx = __hip_atomic_fetch_add(&b.q->next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) sh.idx = __hip_atomic_fetch_add(&b.q->next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        const uint32_t idx = s_idx;
+        const uint32_t idx = sh.idx;
         if (idx >= (uint64_t)b.ntables) break;
         const int32_t t = b.order[idx];
         if (tid == 0) {
-            sa = b.tables[t];
+            sh.a = b.tables[t];
             b.ticks[2 * t] = wall_clock64();
         }
         __syncthreads();
-        wg_build_table<T>(sa, lds_raw, b.wave_lds_doubles, b.learned != 0, tid, nthreads, &s_word);
+        wg_build_table<T>(sh.a, lds_raw, b.wave_lds_doubles, b.learned != 0, tid, nthreads, sh);
         __syncthreads();
         if (tid == 0) b.ticks[2 * t + 1] = wall_clock64();
+    }
+    if (tid == 0) {
+        const uint32_t d = __hip_atomic_fetch_add(&b.q->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (d == gridDim.x - 1) {
+            // Every workgroup has taken its last ticket.
+            __hip_atomic_store(&b.q->next, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&b.q->done, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -299,8 +327,6 @@ hipError_t launch_batch(const atg_batch* b, hipStream_t s) {
     k.ticks = b->d_ticks;
     k.wave_lds_doubles = (int64_t)(psa_lds_bytes(b->Ls_max) / sizeof(double));
     k.learned = b->opts.split_method == ATG_SPLIT_LEARNED;
-    const unsigned rg = (unsigned)std::min<int64_t>(b->ntables, 4096);
-    hipLaunchKernelGGL(batch_reset, dim3(rg), dim3(256), 0, s, (const BuildArgs*)b->d_args, b->ntables, b->d_queue);
     hipLaunchKernelGGL(batch_build<T>, dim3(b->grid), dim3(64 * b->nwaves), b->lds_bytes, s, k);
     return hipGetLastError();
 }
@@ -433,6 +459,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
     if (hipMemcpy(b->d_order, order.data(), sizeof(int32_t) * ntables, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_offsets, offsets, sizeof(int64_t) * (ntables + 1), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(b->d_state, 0, sizeof(BatchState) * ntables) != hipSuccess ||
+        hipMemset(b->d_queue, 0, sizeof(BatchQueue)) != hipSuccess ||
         hipMemset(b->d_ticks, 0, sizeof(uint64_t) * 2 * ntables) != hipSuccess)
         return cleanup(fail(ATG_EDEVICE, "atg_batch_create: upload failed"));
 
@@ -484,8 +511,8 @@ atg_status atg_batch_set_weights(atg_batch* b, const void* w, atg_mem mem, void*
     return ATG_OK;
 }
 
-// The two launches of a build (queue reset, persistent kernel) are captured
-// once into a hipGraph and replayed on the caller's stream.
+// The build launch is captured once into a hipGraph and replayed on the
+// caller's stream (the queue resets itself at the end of every launch).
 atg_status atg_batch_build(atg_batch* b, void* stream) {
     if (!b) return fail(ATG_EINVAL, "atg_batch_build: NULL batch");
     if (!b->weights_set) return fail(ATG_ESTATE, "atg_batch_build: weights not set");

@@ -157,7 +157,32 @@ __device__ Prefix block_prefix(const BuildArgs& a, int64_t s, int lane, bool& ti
 // canonical local-scan aggregates (published as the level-0 block of the
 // dyadic section tree) and the leftovers staged in rank order at the section's
 // own range (lights, then heavies).  Nothing here waits for another section.
-template <typename T, bool AOS>
+// PUBLISH: the block sums are completed by the sections themselves (agent-
+// scope slots, one section per workgroup); otherwise the level-0 sums are
+// plain stores and the caller builds the upper levels after a barrier.
+// P(s) from completed block sums with plain loads (same fold as block_prefix).
+__device__ __forceinline__ Prefix block_prefix_plain(const BuildArgs& a, int64_t s, int lane) {
+    uint64_t c = 0, l = d2u(0.0), h = d2u(0.0);
+    const bool set = lane < 62 && ((s >> lane) & 1);
+    if (set) {
+        const int64_t pos = (s >> (lane + 1)) << (lane + 1);
+        const int64_t idx = level_offset(a.G, lane) + (pos >> lane);
+        c = a.slots[idx];
+        l = a.slots[a.tree_total + idx];
+        h = a.slots[2 * a.tree_total + idx];
+    }
+    const uint64_t bits = __ballot(set);
+    Prefix P{0, 0.0, 0.0};
+    for (int b = 63; b >= 0; --b) {
+        if (!((bits >> b) & 1)) continue;
+        P.c += readlane_u64(c, b);
+        P.l = P.l + u2d(readlane_u64(l, b));
+        P.h = P.h + u2d(readlane_u64(h, b));
+    }
+    return P;
+}
+
+template <typename T, bool AOS, bool PUBLISH>
 __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double* lds_raw, int lane) {
     DevState* st = a.st;
     const T* w = (const T*)a.weights;
@@ -291,9 +316,36 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
             a.stage_i[start + r] = (int32_t)(start + (heavy ? L.Hi(pos) : L.Li(pos)));
         }
     }
-    bool timeout = false;
-    if (lane == 0) block_publish(a, s, aggC, aggL, aggH, timeout);
-    if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
+    if (PUBLISH) {
+        bool timeout = false;
+        if (lane == 0) block_publish(a, s, aggC, aggL, aggH, timeout);
+        if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
+    } else if (lane == 0) {
+        a.slots[s] = aggC;
+        a.slots[a.tree_total + s] = d2u(aggL);
+        a.slots[2 * a.tree_total + s] = d2u(aggH);
+    }
+}
+
+// Upper levels of the dyadic block sums of one table by a whole workgroup,
+// level by level with barriers (the sums block_publish forms, same operands
+// and order: left child + right child).
+__device__ __forceinline__ void wg_block_tree(const BuildArgs& a, int tid, int nthreads) {
+    uint64_t* V = a.slots;
+    const int64_t G = a.G, tt = a.tree_total;
+    int64_t off_child = 0, off_parent = G;
+    for (int b = 1; ((int64_t)1 << b) <= G; ++b) {
+        const int64_t nb = G >> b;
+        for (int64_t c = tid; c < nb; c += nthreads) {
+            const int64_t l = off_child + 2 * c, r = l + 1;
+            V[off_parent + c] = V[l] + V[r];
+            V[tt + off_parent + c] = d2u(u2d(V[tt + l]) + u2d(V[tt + r]));
+            V[2 * tt + off_parent + c] = d2u(u2d(V[2 * tt + l]) + u2d(V[2 * tt + r]));
+        }
+        __syncthreads();
+        off_child = off_parent;
+        off_parent += nb;
+    }
 }
 
 // Partition pass 2, section s by one wave: the section prefix P(s) from the
@@ -301,14 +353,24 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
 // P(s) + local (the canonical chunked scan recomputed in the same order),
 // the split certificate, and for the last section the totals, sentinels and
 // split inputs.
-__device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int lane) {
+// BATCH: the block sums were completed before a workgroup barrier (plain
+// loads, no waiting) and the certificate extremes are folded into ext_l /
+// ext_h for the caller instead of the table state.
+template <bool BATCH>
+__device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int lane, double& ext_l_out,
+                                              double& ext_h_out) {
     DevState* st = a.st;
     const int64_t start = min(a.n, s * a.Ls);
     bool timeout = false;
-    uint64_t c0, l0, h0;
-    block_load(a, s, c0, l0, h0, timeout);  // this section's own counts
+    uint64_t c0;
+    if (BATCH) {
+        c0 = a.slots[s];
+    } else {
+        uint64_t l0, h0;
+        block_load(a, s, c0, l0, h0, timeout);  // this section's own counts
+    }
     const int mL = (int)(uint32_t)c0, mH = (int)(c0 >> 32);
-    const Prefix P = block_prefix(a, s, lane, timeout);
+    const Prefix P = BATCH ? block_prefix_plain(a, s, lane) : block_prefix(a, s, lane, timeout);
     const int64_t goffL = (int64_t)(uint32_t)P.c, goffH = (int64_t)(P.c >> 32);
     double ext_l = 0.0, ext_h = __builtin_inf();
     for (int hv = 0; hv < 2; ++hv) {
@@ -350,13 +412,16 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
         ext_l = fmax(ext_l, shfl_xor_d(ext_l, m));
         ext_h = fmin(ext_h, shfl_xor_d(ext_h, m));
     }
-    if (lane == 0) {
+    if (BATCH) {
+        if (mL > 0) ext_l_out = fmax(ext_l_out, ext_l);
+        if (mH > 0) ext_h_out = fmin(ext_h_out, ext_h);
+    } else if (lane == 0) {
         if (mL > 0) atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)d2u(ext_l));
         if (mH > 0) atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)d2u(ext_h));
     }
     if (s == a.G - 1) {
         // Totals P(G), sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
-        const Prefix Tot = block_prefix(a, a.G, lane, timeout);
+        const Prefix Tot = BATCH ? block_prefix_plain(a, a.G, lane) : block_prefix(a, a.G, lane, timeout);
         if (lane == 0) {
             const int64_t nLr = (int64_t)(uint32_t)Tot.c, nHr = (int64_t)(Tot.c >> 32);
             a.Li[nLr] = (int32_t)(a.n + 1); a.Lw[nLr] = 0.0; a.Lp[nLr] = Tot.l;
@@ -372,7 +437,7 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
             st->wToSplit = Tot.l + Tot.h;
         }
     }
-    if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
+    if (!BATCH && __ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
 }
 
 
@@ -431,8 +496,10 @@ __device__ __forceinline__ int64_t knot_k(int64_t m, int64_t p, int64_t K) { ret
 
 // Knot m (0 <= m <= K) of the learned split index: the exact split at
 // k_m = floor(m p / K), by bisection.
-__device__ __forceinline__ void knot_split(const BuildArgs& a, int64_t m) {
+// Returns 1 when the bisection fell back to its clamp (the caller counts it).
+__device__ __forceinline__ uint32_t knot_split(const BuildArgs& a, int64_t m) {
     DevState* st = a.st;
+    uint32_t fb = 0;
     const int64_t p = st->p, numL = st->numL, numH = st->numH;
     const int64_t K = min<int64_t>(a.learned_segments, p);
     const int64_t k = knot_k(m, p, K);
@@ -442,16 +509,19 @@ __device__ __forceinline__ void knot_split(const BuildArgs& a, int64_t m) {
     } else if (k == p) {
         sp = Split{(int32_t)(numL - 1), (int32_t)(numH - 1), 0.0};
     } else if (!bisect_split(k, p, st->nToSplit, st->wToSplit, a.Lp, numL, a.Hp, a.Hw, numH, sp)) {
-        atomicAdd((unsigned long long*)&st->split_fallbacks, 1ull);
+        fb = 1;
     }
     a.splits[k] = sp;
     a.knot_j[m] = sp.j;
+    return fb;
 }
 
 // Split k (0 < k < p, not a knot) from the model: interpolate between the
 // knots, gallop, bisect inside the bracket; full bisection when the
 // certificate is missing or the bracket search fails.
-__device__ __forceinline__ void learned_split(const BuildArgs& a, int64_t k) {
+// Returns bit 0: the bisection fell back to its clamp; bit 1: the learned
+// search fell back to full bisection (the caller counts them).
+__device__ __forceinline__ uint32_t learned_split(const BuildArgs& a, int64_t k) {
     DevState* st = a.st;
     const int64_t p = st->p, numL = st->numL, numH = st->numH;
     const int64_t K = min<int64_t>(a.learned_segments, p);
@@ -460,15 +530,12 @@ __device__ __forceinline__ void learned_split(const BuildArgs& a, int64_t k) {
     while (m < K && knot_k(m + 1, p, K) <= k) ++m;
     while (m > 0 && knot_k(m, p, K) > k) --m;
     const int64_t km = knot_k(m, p, K), km1 = knot_k(m + 1, p, K);
-    if (k == km) return;  // a knot: written by split_knots
+    if (k == km) return 0;  // a knot: written by knot_split
     const int64_t nToSplit = st->nToSplit;
     const double wToSplit = st->wToSplit;
     double dl, dh;
     {
-        // Atomic loads: the certificate was accumulated with L2 atomics, possibly
-        // earlier in the same launch (batched build).
-        const uint64_t bl = __hip_atomic_load(&st->cert_max_dL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t bh = __hip_atomic_load(&st->cert_min_dH, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t bl = st->cert_max_dL, bh = st->cert_min_dH;
         dl = u2d(bl);
         dh = u2d(bh);
     }
@@ -527,12 +594,13 @@ __device__ __forceinline__ void learned_split(const BuildArgs& a, int64_t k) {
             }
         }
     }
+    uint32_t flags = 0;
     if (!ok) {
-        if (!bisect_split(k, p, nToSplit, wToSplit, a.Lp, numL, a.Hp, a.Hw, numH, out))
-            atomicAdd((unsigned long long*)&st->split_fallbacks, 1ull);
-        atomicAdd((unsigned long long*)&st->learned_fallbacks, 1ull);
+        if (!bisect_split(k, p, nToSplit, wToSplit, a.Lp, numL, a.Hp, a.Hw, numH, out)) flags |= 1u;
+        flags |= 2u;
     }
     a.splits[k] = out;
+    return flags;
 }
 
 }  // namespace atg

@@ -23,7 +23,7 @@ __global__ __launch_bounds__(256) void split_knots(BuildArgs a) {
     if (st->error || st->degenerate) return;
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m > min<int64_t>(a.learned_segments, st->p)) return;
-    knot_split(a, m);
+    if (knot_split(a, m)) atomicAdd((unsigned long long*)&st->split_fallbacks, 1ull);
 }
 
 __global__ __launch_bounds__(256) void split_learned(BuildArgs a) {
@@ -31,7 +31,9 @@ __global__ __launch_bounds__(256) void split_learned(BuildArgs a) {
     if (st->error || st->degenerate) return;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (k >= st->p) return;
-    learned_split(a, k);
+    const uint32_t f = learned_split(a, k);
+    if (f & 1u) atomicAdd((unsigned long long*)&st->split_fallbacks, 1ull);
+    if (f & 2u) atomicAdd((unsigned long long*)&st->learned_fallbacks, 1ull);
 }
 
 hipError_t launch_split_learned(const BuildArgs& a, hipStream_t s) {

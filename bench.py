@@ -172,9 +172,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=100_000_000)
     ap.add_argument("--samples", type=int, default=1_000_000_000)
-    ap.add_argument("--section-size", type=int, default=0, help="PSA+ section (0 = reference min(1000, N/10))")
+    ap.add_argument("--section-size", type=int, default=2048,
+                    help="PSA+ section (0 = the reference's min(1000, N/10))")
     ap.add_argument("--chains", type=int, default=16)
-    ap.add_argument("--num-splits", type=int, default=0, help="p (0 = reference AUTO, 320 per CU)")
+    ap.add_argument("--num-splits", type=int, default=2621440, help="p (0 = the reference's AUTO, 320 per CU)")
     ap.add_argument("--split", choices=["bisect", "learned"], default="bisect")
     ap.add_argument("--learned-segments", type=int, default=4096)
     ap.add_argument("--sample-section", type=int, default=4096)
