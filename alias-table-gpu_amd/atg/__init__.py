@@ -187,13 +187,13 @@ class Table:
 
 
 class Batch:
-    """Many independent tables built in one persistent launch (atg_batch_*).
+    """Many independent tables built together, one launch per phase (atg_batch_*).
 
     Table t owns weights[offsets[t]:offsets[t+1]] and is built exactly as a
     `Table` with the same options would build it alone."""
 
     def __init__(self, offsets, dtype="f64", psa_plus=True, chains=16, section_size=0, num_splits=0,
-                 split="bisect", learned_segments=4096, waves=0):
+                 split="bisect", learned_segments=4096):
         o = default_opts()
         o.psa_plus = 1 if psa_plus else 0
         o.chains = int(chains)
@@ -201,7 +201,6 @@ class Batch:
         o.num_splits = int(num_splits)
         o.split_method = ATG_SPLIT_LEARNED if split == "learned" else ATG_SPLIT_BISECT
         o.learned_segments = int(learned_segments)
-        o.batch_waves = int(waves)
         self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         self.ntables = self.offsets.size - 1
         self.total = int(self.offsets[-1])

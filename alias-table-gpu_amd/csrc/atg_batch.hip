@@ -1,23 +1,14 @@
-// Batched construction of many independent alias tables in one persistent
-// launch (no counterpart in the reference, which builds one table per object:
-// AliasTableSplitGpu.cu:388-468).
+// Batched construction of many independent alias tables (no counterpart in
+// the reference, which builds one table per object: AliasTableSplitGpu.cu:388-468).
 //
-// One workgroup builds one table from start to end; the workgroups take
-// tables from a queue in longest-first order, so the largest tables start
-// first and the grid drains when the queue is empty.  Inside a table the
-// phases are the single-table phases with workgroup barriers instead of
-// kernel boundaries, and every phase calls the same device functions with
-// the same association, so a table built here is bit-identical to the same
-// weights built alone with the same options:
-//   W       : section sums by the waves, then the 64-ary tree level by level;
-//   PSA+    : wave w takes sections w, w + nw, ...: pass 1 (partition,
-//             chains, staged leftovers, level-0 block sums), a barrier, the
-//             upper block sums level by level, then pass 2 (section
-//             prefixes, leftover lists, certificate);
-//   split   : bisection or the learned index per k;
-//   pack    : one chain per k.
-// Device timestamps (the 100 MHz wall clock) at the start and end of every
-// table give per-table build latencies.
+// Every phase is one launch over all tables: a block finds its table and its
+// block within the table from a block->table map built at creation, then runs
+// exactly the device code of the single-table kernels (the same launch shape
+// per table: 16-wave sum blocks, one wave per section, one wave per 64 pack
+// chains).  A table built here is therefore bit-identical to the same weights
+// built alone with the same options.  Per-table latencies come from the
+// 100 MHz device clock: the first block of the table's sum phase to the last
+// block of its pack.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,26 +22,24 @@
 
 namespace atg {
 
-struct BatchQueue {
-    uint32_t next;   // next table ticket
-    uint32_t done;   // workgroups that found the queue empty
-    uint32_t pad[62];
-};
-
 // Per-table build state on its own 256-byte line pair, so that no cache line
-// is shared between tables that different workgroups build.
+// is shared between tables.
 struct alignas(256) BatchState {
     DevState s;
 };
 
-struct BatchBuildArgs {
-    const BuildArgs* tables;  // per-table arguments (device)
-    const int32_t* order;     // longest-first table order
+// Global block b of a phase belongs to table tab[b]; its block within the
+// table is b - base[tab[b]].
+struct SegMap {
+    const int32_t* tab;
+    const int64_t* base;
+    int64_t blocks;
+};
+
+struct SegArgs {
+    const BuildArgs* tables;
     int64_t ntables;
-    BatchQueue* q;
-    uint64_t* ticks;          // [2 t] start, [2 t + 1] end
-    int64_t wave_lds_doubles; // LDS stride of one wave's section buffer
-    int32_t learned;
+    uint64_t* ticks;  // [2 t] first start, [2 t + 1] last end (device clock)
 };
 
 struct BatchSampleArgs {
@@ -63,178 +52,107 @@ struct BatchSampleArgs {
     int32_t* out;             // ntables * per_table
 };
 
-// Workgroup-shared scratch of one table build.
-struct TableLDS {
-    BuildArgs a;
-    uint32_t idx;
-    uint32_t flags;
-    uint32_t split_fallbacks;
-    uint32_t learned_fallbacks;
-    uint64_t cert_l, cert_h;  // bit patterns (non-negative doubles order like integers)
-};
-
-// W of one table: section sums (lane l adds items l, l + 64, ...), then the
-// 64-ary tree over the section sums, 64 values per wave and level, zero-padded
-// (the association of sum_sections).  Returns the error bits.
-template <typename T>
-__device__ __forceinline__ uint32_t wg_weight_sum(const BuildArgs& a, int tid, int nthreads, TableLDS& sh) {
-    const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
-    const T* w = (const T*)a.weights;
-    for (int64_t s = wave; s < a.G; s += nw) {
-        const double sum = section_sum<T>(w, min(a.n, s * a.Ls), min(a.n, (s + 1) * a.Ls), lane);
-        if (lane == 0) a.section_sums[s] = sum;
-    }
-    __syncthreads();
-    double* bufs[2] = {a.tree_scratch, a.tree_scratch + a.tree_scratch_half};
-    const double* src = a.section_sums;
-    int cur = 0;
-    int64_t count = a.G;
-    while (true) {
-        const int64_t groups = (count + 63) / 64;
-        for (int64_t g = wave; g < groups; g += nw) {
-            const int64_t i = g * 64 + lane;
-            const double r = wave_tree_sum((i < count) ? src[i] : 0.0);
-            if (lane == 0) bufs[cur][g] = r;
-        }
-        __syncthreads();
-        src = bufs[cur];
-        cur ^= 1;
-        count = groups;
-        if (count == 1) break;
-    }
-    if (tid == 0) {
-        const double W = src[0];
-        DevState* st = a.st;
-        uint32_t err = 0;
-        if (W != W) err = ERR_BAD_WEIGHT;
-        else if (!(W > 0.0) || W == __builtin_inf()) err = ERR_BAD_SUM;
-        if (a.Ls > a.Ls_max) err |= ERR_SECTION_TOO_LONG;
-        st->W = W;
-        st->W_N = W / (double)a.n;
-        st->error = err;
-        st->degenerate = 0;
-        st->p = 0;
-        sh.flags = err;
-        sh.split_fallbacks = 0;
-        sh.learned_fallbacks = 0;
-        sh.cert_l = 0ull;                   // +0.0
-        sh.cert_h = 0x7FF0000000000000ull;  // +inf
-    }
-    __syncthreads();
-    return sh.flags;
+// Blocks of the degenerate-table fill per table (a grid-stride loop covers
+// the rest; most tables are not degenerate and leave at once).
+__host__ __device__ inline int64_t degenerate_blocks(int64_t n) {
+    const int64_t b = (n + 2 + 255) / 256;
+    return b < 8 ? b : 8;
 }
 
-// One table by one workgroup.  Communication between the waves goes through
-// plain global memory ordered by workgroup barriers and through LDS; there is
-// no waiting on another wave's progress and no global atomic.
-template <typename T>
-__device__ __forceinline__ void wg_build_table(const BuildArgs& a, double* lds_raw, int64_t wave_lds_doubles,
-                                               bool learned, int tid, int nthreads, TableLDS& sh) {
-    const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
-    DevState* st = a.st;
-    if (wg_weight_sum<T>(a, tid, nthreads, sh)) return;
-
-    // Partition pass 1 per section, the dyadic block sums, pass 2 per section.
-    double* mine = lds_raw + (int64_t)wave * wave_lds_doubles;
-    for (int64_t s = wave; s < a.G; s += nw) psa_local<T, true, false>(a, s, mine, lane);
-    __syncthreads();
-    wg_block_tree(a, tid, nthreads);
-    double ext_l = 0.0, ext_h = __builtin_inf();
-    for (int64_t s = wave; s < a.G; s += nw) place_section<true>(a, s, lane, ext_l, ext_h);
-    if (lane == 0) {
-        atomicMax((unsigned long long*)&sh.cert_l, (unsigned long long)d2u(ext_l));
-        atomicMin((unsigned long long*)&sh.cert_h, (unsigned long long)d2u(ext_h));
-    }
-    __syncthreads();
-    if (tid == 0) {
-        st->cert_max_dL = sh.cert_l;
-        st->cert_min_dH = sh.cert_h;
-        sh.flags = st->degenerate;
-    }
-    __syncthreads();
-    if (sh.flags) {
-        // One leftover class is empty: every leftover row carries its own mass.
-        const int64_t nLr = st->numL - 1, nHr = st->numH - 1;
-        for (int64_t t = tid; t < nLr + nHr; t += nthreads) {
-            const int32_t item = (t < nLr) ? a.Li[t] : a.Hi[t - nLr];
-            const double w = (t < nLr) ? a.Lw[t] : a.Hw[t - nLr];
-            store_row<true>(a, item, w, item);
-        }
-        if (tid == 0) {
-            st->split_fallbacks = 0;
-            st->learned_fallbacks = 0;
-        }
-        return;
-    }
-
-    // Split.
-    const int64_t p = st->p, numL = st->numL, numH = st->numH;
-    uint32_t sfb = 0, lfb = 0;
-    if (learned) {
-        const int64_t K = min<int64_t>(a.learned_segments, p);
-        for (int64_t m = tid; m <= K; m += nthreads) sfb += knot_split(a, m);
-        __syncthreads();
-        for (int64_t k = 1 + tid; k < p; k += nthreads) {
-            const uint32_t f = learned_split(a, k);
-            sfb += f & 1u;
-            lfb += (f >> 1) & 1u;
-        }
-    } else {
-        if (tid == 0) {
-            a.splits[0] = Split{0, 0, 0.0};
-            a.splits[p] = Split{(int32_t)(numL - 1), (int32_t)(numH - 1), 0.0};
-        }
-        for (int64_t k = 1 + tid; k < p; k += nthreads) {
-            Split sp;
-            if (!bisect_split(k, p, st->nToSplit, st->wToSplit, a.Lp, numL, a.Hp, a.Hw, numH, sp)) ++sfb;
-            a.splits[k] = sp;
-        }
-    }
-    if (sfb) atomicAdd(&sh.split_fallbacks, sfb);
-    if (lfb) atomicAdd(&sh.learned_fallbacks, lfb);
-    __syncthreads();
-    if (tid == 0) {
-        st->split_fallbacks = sh.split_fallbacks;
-        st->learned_fallbacks = sh.learned_fallbacks;
-    }
-
-    // Pack: one chain per section of the split.
-    const double W_N = st->W_N;
-    const GlobalLists l{a.Li, a.Hi, a.Lw, a.Hw};
-    for (int64_t k = 1 + tid; k <= p; k += nthreads) pack_chain<true>(a, l, a.splits[k - 1], a.splits[k], W_N);
+__device__ __forceinline__ void seg_locate(const SegMap& m, int64_t& t, int64_t& blk) {
+    t = m.tab[blockIdx.x];
+    blk = (int64_t)blockIdx.x - m.base[t];
 }
 
-// Persistent: every workgroup takes tables until the queue is empty; the last
-// workgroup to find it empty resets the queue for the next launch (so a
-// replayed graph needs no reset node).
+__device__ __forceinline__ void tick_end(const SegArgs& g, int64_t t) {
+    if (threadIdx.x == 0)
+        atomicMax((unsigned long long*)&g.ticks[2 * t + 1], (unsigned long long)wall_clock64());
+}
+
+__global__ __launch_bounds__(256) void seg_reset_ticks(SegArgs g) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < g.ntables; t += (int64_t)gridDim.x * blockDim.x) {
+        g.ticks[2 * t] = ~0ull;
+        g.ticks[2 * t + 1] = 0ull;
+    }
+}
+
 template <typename T>
-__global__ __launch_bounds__(1024) void batch_build(BatchBuildArgs b) {
+__global__ __launch_bounds__(1024) void seg_sum(SegArgs g, SegMap m) {
+    __shared__ double sh[64];
+    __shared__ uint32_t sh_last;
+    int64_t t, blk;
+    seg_locate(m, t, blk);
+    if (threadIdx.x == 0) atomicMin((unsigned long long*)&g.ticks[2 * t], (unsigned long long)wall_clock64());
+    const BuildArgs& a = g.tables[t];
+    sum_block<T>(a, blk, (a.G + 63) / 64, sh, sh_last);
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void seg_local(SegArgs g, SegMap m) {
     extern __shared__ double lds_raw[];
-    __shared__ TableLDS sh;
-    const int tid = threadIdx.x, nthreads = blockDim.x;
-    while (true) {
-        if (tid == 0) sh.idx = __hip_atomic_fetch_add(&b.q->next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        const uint32_t idx = sh.idx;
-        if (idx >= (uint64_t)b.ntables) break;
-        const int32_t t = b.order[idx];
-        if (tid == 0) {
-            sh.a = b.tables[t];
-            b.ticks[2 * t] = wall_clock64();
-        }
-        __syncthreads();
-        wg_build_table<T>(sh.a, lds_raw, b.wave_lds_doubles, b.learned != 0, tid, nthreads, sh);
-        __syncthreads();
-        if (tid == 0) b.ticks[2 * t + 1] = wall_clock64();
-    }
-    if (tid == 0) {
-        const uint32_t d = __hip_atomic_fetch_add(&b.q->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (d == gridDim.x - 1) {
-            // Every workgroup has taken its last ticket.
-            __hip_atomic_store(&b.q->next, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&b.q->done, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    int64_t t, blk;
+    seg_locate(m, t, blk);
+    const BuildArgs& a = g.tables[t];
+    if (a.st->error) return;
+    psa_local<T, true, true>(a, blk, lds_raw, (int)threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void seg_place(SegArgs g, SegMap m) {
+    int64_t t, blk;
+    seg_locate(m, t, blk);
+    const BuildArgs& a = g.tables[t];
+    if (a.st->error) return;
+    const int64_t s = blk * 4 + (threadIdx.x >> 6);
+    if (s >= a.G) return;
+    double unused_l = 0.0, unused_h = 0.0;
+    place_section<false>(a, s, (int)(threadIdx.x & 63), unused_l, unused_h);
+}
+
+__global__ __launch_bounds__(256) void seg_degenerate(SegArgs g, SegMap m) {
+    int64_t t, blk;
+    seg_locate(m, t, blk);
+    const BuildArgs& a = g.tables[t];
+    tick_end(g, t);
+    degenerate_block<true>(a, blk, degenerate_blocks(a.n));
+}
+
+__global__ __launch_bounds__(256) void seg_split(SegArgs g, SegMap m) {
+    int64_t t, blk;
+    seg_locate(m, t, blk);
+    split_bisect_block(g.tables[t], blk);
+}
+
+__global__ __launch_bounds__(256) void seg_knots(SegArgs g, SegMap m) {
+    int64_t t, blk;
+    seg_locate(m, t, blk);
+    const BuildArgs& a = g.tables[t];
+    DevState* st = a.st;
+    if (st->error || st->degenerate) return;
+    const int64_t mm = blk * blockDim.x + threadIdx.x;
+    if (mm > min<int64_t>(a.learned_segments, st->p)) return;
+    if (knot_split(a, mm)) atomicAdd((unsigned long long*)&st->split_fallbacks, 1ull);
+}
+
+__global__ __launch_bounds__(256) void seg_learned(SegArgs g, SegMap m) {
+    int64_t t, blk;
+    seg_locate(m, t, blk);
+    const BuildArgs& a = g.tables[t];
+    DevState* st = a.st;
+    if (st->error || st->degenerate) return;
+    const int64_t k = blk * blockDim.x + threadIdx.x + 1;
+    if (k >= st->p) return;
+    const uint32_t f = learned_split(a, k);
+    if (f & 1u) atomicAdd((unsigned long long*)&st->split_fallbacks, 1ull);
+    if (f & 2u) atomicAdd((unsigned long long*)&st->learned_fallbacks, 1ull);
+}
+
+__global__ __launch_bounds__(64) void seg_pack(SegArgs g, SegMap m) {
+    extern __shared__ double lds_raw[];
+    int64_t t, blk;
+    seg_locate(m, t, blk);
+    tick_end(g, t);
+    pack_staged_block<true>(g.tables[t], blk, lds_raw);
+    tick_end(g, t);
 }
 
 // Basic alias sampling of every table: sample k of table t uses the Philox
@@ -264,6 +182,8 @@ __global__ __launch_bounds__(256) void batch_sample(BatchSampleArgs a) {
 
 using namespace atg;
 
+enum { MAP_SUM, MAP_LOCAL, MAP_PLACE, MAP_DEG, MAP_SPLIT, MAP_KNOTS, MAP_PACK, MAP_COUNT };
+
 struct atg_batch_s {
     int64_t ntables = 0, total = 0;
     atg_dtype dtype = ATG_F64;
@@ -272,27 +192,22 @@ struct atg_batch_s {
     std::vector<int64_t> offsets;
     std::vector<BuildArgs> args;        // host copy (pointers into the pools below)
     int64_t Ls_max = 1;
-    int nwaves = 4;
-    int grid = 0;
-    size_t lds_bytes = 0;
     double wall_khz = 100000.0;
+    SegMap maps[MAP_COUNT] = {};
     // device memory
     void* d_weights_owned = nullptr;
     const void* d_weights = nullptr;
     BuildArgs* d_args = nullptr;
-    int32_t* d_order = nullptr;
     int64_t* d_offsets = nullptr;
-    BatchQueue* d_queue = nullptr;
     uint64_t* d_ticks = nullptr;
     BatchState* d_state = nullptr;
     Row* d_rows = nullptr;
-    double* d_dpool = nullptr;          // section sums, tree, L/H weights and prefixes
+    double* d_dpool = nullptr;          // section sums, tree, L/H weights and prefixes, staging
     uint64_t* d_slots = nullptr;
-    int32_t* d_ipool = nullptr;         // L/H items, knots
+    int32_t* d_ipool = nullptr;         // L/H items, staging, knots
     Split* d_splits = nullptr;
-    hipStream_t capture = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    int32_t* d_maptab = nullptr;        // block->table maps of all phases
+    int64_t* d_mapbase = nullptr;       // first block of every table, all phases
     bool weights_set = false, built = false;
 };
 
@@ -301,11 +216,8 @@ namespace {
 inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 void free_batch(atg_batch* b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    if (b->capture) (void)hipStreamDestroy(b->capture);
-    void* ptrs[] = {b->d_weights_owned, b->d_args, b->d_order, b->d_offsets, b->d_queue, b->d_ticks, b->d_state,
-                    b->d_rows, b->d_dpool, b->d_slots, b->d_ipool, b->d_splits};
+    void* ptrs[] = {b->d_weights_owned, b->d_args, b->d_offsets, b->d_ticks, b->d_state, b->d_rows,
+                    b->d_dpool, b->d_slots, b->d_ipool, b->d_splits, b->d_maptab, b->d_mapbase};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -319,15 +231,22 @@ void bind_weights(atg_batch* b) {
 
 template <typename T>
 hipError_t launch_batch(const atg_batch* b, hipStream_t s) {
-    BatchBuildArgs k{};
-    k.tables = b->d_args;
-    k.order = b->d_order;
-    k.ntables = b->ntables;
-    k.q = b->d_queue;
-    k.ticks = b->d_ticks;
-    k.wave_lds_doubles = (int64_t)(psa_lds_bytes(b->Ls_max) / sizeof(double));
-    k.learned = b->opts.split_method == ATG_SPLIT_LEARNED;
-    hipLaunchKernelGGL(batch_build<T>, dim3(b->grid), dim3(64 * b->nwaves), b->lds_bytes, s, k);
+    SegArgs g{b->d_args, b->ntables, b->d_ticks};
+    const SegMap* M = b->maps;
+    auto grid = [](const SegMap& m) { return dim3((unsigned)m.blocks); };
+    hipLaunchKernelGGL(seg_reset_ticks, dim3((unsigned)std::min<int64_t>((b->ntables + 255) / 256, 4096)), dim3(256),
+                       0, s, g);
+    hipLaunchKernelGGL(seg_sum<T>, grid(M[MAP_SUM]), dim3(1024), 0, s, g, M[MAP_SUM]);
+    hipLaunchKernelGGL(seg_local<T>, grid(M[MAP_LOCAL]), dim3(64), psa_lds_bytes(b->Ls_max), s, g, M[MAP_LOCAL]);
+    hipLaunchKernelGGL(seg_place, grid(M[MAP_PLACE]), dim3(256), 0, s, g, M[MAP_PLACE]);
+    hipLaunchKernelGGL(seg_degenerate, grid(M[MAP_DEG]), dim3(256), 0, s, g, M[MAP_DEG]);
+    if (b->opts.split_method == ATG_SPLIT_LEARNED) {
+        hipLaunchKernelGGL(seg_knots, grid(M[MAP_KNOTS]), dim3(256), 0, s, g, M[MAP_KNOTS]);
+        hipLaunchKernelGGL(seg_learned, grid(M[MAP_SPLIT]), dim3(256), 0, s, g, M[MAP_SPLIT]);
+    } else {
+        hipLaunchKernelGGL(seg_split, grid(M[MAP_SPLIT]), dim3(256), 0, s, g, M[MAP_SPLIT]);
+    }
+    hipLaunchKernelGGL(seg_pack, grid(M[MAP_PACK]), dim3(64), (size_t)PACK_CAP * 24, s, g, M[MAP_PACK]);
     return hipGetLastError();
 }
 
@@ -354,15 +273,12 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         return fail(ATG_EINVAL, "atg_batch_create: unknown split method %d", o.split_method);
     if (o.learned_segments <= 0) o.learned_segments = 4096;
     const int64_t p_auto = 32LL * device_cus() * 10;
-    int nwaves = o.batch_waves > 0 ? o.batch_waves : 4;
-    if (nwaves < 1 || nwaves > 16) return fail(ATG_EINVAL, "atg_batch_create: %d waves per table", nwaves);
 
     atg_batch* b = new atg_batch_s();
     b->ntables = ntables;
     b->total = offsets[ntables];
     b->dtype = dtype;
     b->opts = o;
-    b->nwaves = nwaves;
     b->offsets.assign(offsets, offsets + ntables + 1);
     b->args.resize((size_t)ntables);
     (void)hipGetDevice(&b->device);
@@ -406,9 +322,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         }                                                                                         \
     } while (0)
     BALLOC(b->d_args, ntables);
-    BALLOC(b->d_order, ntables);
     BALLOC(b->d_offsets, ntables + 1);
-    BALLOC(b->d_queue, 1);
     BALLOC(b->d_ticks, 2 * ntables);
     BALLOC(b->d_state, ntables);
     BALLOC(b->d_rows, b->total);
@@ -450,35 +364,71 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         a.splits = b->d_splits + osp[t];
         a.learned_segments = o.learned_segments;
     }
-    // Longest tables first (ties by index).
-    std::vector<int32_t> order((size_t)ntables);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        return offsets[x + 1] - offsets[x] > offsets[y + 1] - offsets[y];
-    });
-    if (hipMemcpy(b->d_order, order.data(), sizeof(int32_t) * ntables, hipMemcpyHostToDevice) != hipSuccess ||
+    // Block->table maps of the phases (blocks per table as the single-table
+    // launchers use them).
+    auto blocks_of = [&](int ph, int64_t t) -> int64_t {
+        const Sizes& z = sz[t];
+        const int64_t n = offsets[t + 1] - offsets[t];
+        const int64_t K = std::min<int64_t>(o.learned_segments, z.p_req);
+        switch (ph) {
+            case MAP_SUM: return (z.G + 63) / 64;
+            case MAP_LOCAL: return z.G;
+            case MAP_PLACE: return (z.G + 3) / 4;
+            case MAP_DEG: return degenerate_blocks(n);
+            case MAP_SPLIT: return std::max<int64_t>(1, (z.p_req - 1 + 255) / 256);
+            case MAP_KNOTS: return (K + 1 + 255) / 256;
+            default: return (z.p_req + 63) / 64;  // MAP_PACK
+        }
+    };
+    std::vector<int64_t> base((size_t)MAP_COUNT * ntables);
+    int64_t totblocks[MAP_COUNT] = {};
+    for (int ph = 0; ph < MAP_COUNT; ++ph)
+        for (int64_t t = 0; t < ntables; ++t) {
+            base[(size_t)ph * ntables + t] = totblocks[ph];
+            totblocks[ph] += blocks_of(ph, t);
+        }
+    int64_t alltab = 0;
+    for (int ph = 0; ph < MAP_COUNT; ++ph) {
+        if (totblocks[ph] > 0x7FFFFFFF)
+            return cleanup(fail(ATG_EINVAL, "atg_batch_create: %lld blocks in one phase", (long long)totblocks[ph]));
+        alltab += totblocks[ph];
+    }
+    std::vector<int32_t> tab((size_t)alltab);
+    {
+        int64_t pos = 0;
+        for (int ph = 0; ph < MAP_COUNT; ++ph)
+            for (int64_t t = 0; t < ntables; ++t) {
+                const int64_t nb = blocks_of(ph, t);
+                for (int64_t q = 0; q < nb; ++q) tab[(size_t)(pos + q)] = (int32_t)t;
+                pos += nb;
+            }
+    }
+    if (dalloc(&b->d_maptab, alltab) != hipSuccess || dalloc(&b->d_mapbase, base.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        return cleanup(fail(ATG_ENOMEM, "atg_batch_create: block maps"));
+    }
+    {
+        int64_t pos = 0;
+        for (int ph = 0; ph < MAP_COUNT; ++ph) {
+            b->maps[ph].tab = b->d_maptab + pos;
+            b->maps[ph].base = b->d_mapbase + (size_t)ph * ntables;
+            b->maps[ph].blocks = totblocks[ph];
+            pos += totblocks[ph];
+        }
+    }
+    if (hipMemcpy(b->d_maptab, tab.data(), sizeof(int32_t) * alltab, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_mapbase, base.data(), sizeof(int64_t) * base.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_offsets, offsets, sizeof(int64_t) * (ntables + 1), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(b->d_state, 0, sizeof(BatchState) * ntables) != hipSuccess ||
-        hipMemset(b->d_queue, 0, sizeof(BatchQueue)) != hipSuccess ||
         hipMemset(b->d_ticks, 0, sizeof(uint64_t) * 2 * ntables) != hipSuccess)
         return cleanup(fail(ATG_EDEVICE, "atg_batch_create: upload failed"));
-
-    // Launch shape: as many resident workgroups as LDS and registers allow.
-    b->lds_bytes = (size_t)b->nwaves * psa_lds_bytes(b->Ls_max);
-    const void* fn = dtype == ATG_F32 ? (const void*)batch_build<float> : (const void*)batch_build<double>;
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds_bytes) != hipSuccess)
-        return cleanup(fail(ATG_EDEVICE, "atg_batch_create: LDS limit of %zu bytes", b->lds_bytes));
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * b->nwaves, b->lds_bytes) != hipSuccess ||
-        per_cu < 1)
-        return cleanup(fail(ATG_EDEVICE, "atg_batch_create: no resident workgroup fits (%zu B LDS)", b->lds_bytes));
-    const int64_t g = (int64_t)per_cu * device_cus();
-    b->grid = (int)std::min<int64_t>(g, ntables);
+    const size_t lds = psa_lds_bytes(b->Ls_max);
+    const void* fn = dtype == ATG_F32 ? (const void*)seg_local<float> : (const void*)seg_local<double>;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return cleanup(fail(ATG_EDEVICE, "atg_batch_create: LDS limit of %zu bytes", lds));
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0)
         b->wall_khz = khz;
-    if (hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking) != hipSuccess)
-        return cleanup(fail(ATG_EDEVICE, "atg_batch_create: stream"));
     *out = b;
     return ATG_OK;
 }
@@ -511,22 +461,12 @@ atg_status atg_batch_set_weights(atg_batch* b, const void* w, atg_mem mem, void*
     return ATG_OK;
 }
 
-// The build launch is captured once into a hipGraph and replayed on the
-// caller's stream (the queue resets itself at the end of every launch).
+// One launch per phase for all tables, on the caller's stream.
 atg_status atg_batch_build(atg_batch* b, void* stream) {
     if (!b) return fail(ATG_EINVAL, "atg_batch_build: NULL batch");
     if (!b->weights_set) return fail(ATG_ESTATE, "atg_batch_build: weights not set");
-    if (!b->exec) {
-        HIPCHK(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
-        const hipError_t e = b->dtype == ATG_F32 ? launch_batch<float>(b, b->capture) : launch_batch<double>(b, b->capture);
-        hipGraph_t g = nullptr;
-        const hipError_t e2 = hipStreamEndCapture(b->capture, &g);
-        HIPCHK(e);
-        HIPCHK(e2);
-        b->graph = g;
-        HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    }
-    HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(b->dtype == ATG_F32 ? launch_batch<float>(b, s) : launch_batch<double>(b, s));
     b->built = true;
     return ATG_OK;
 }

@@ -119,15 +119,15 @@ def bench_learned(args, stream):
 
 
 def bench_batch(args, stream, name, ntables, lo, hi, dist, dtype, split, per_table, param=0.0):
-    """Batched build of many tables in one persistent launch (replayed hipGraph),
-    plus `per_table` samples from every table; per-table latencies from
-    device timestamps."""
+    """Batched build of many tables (one launch per phase over all tables), plus
+    `per_table` samples from every table; per-table latencies from device
+    timestamps."""
     import numpy as np
     import torch
     import atg
     offsets, w = atg.generate_batch(dist, ntables, lo, hi, seed=42, param=param, dtype=dtype)
     total = int(offsets[-1])
-    b = atg.Batch(offsets, dtype=dtype, split=split, waves=args.batch_waves)
+    b = atg.Batch(offsets, dtype=dtype, split=split)
     b.set_weights(w, stream)
     del w
     out = torch.empty(ntables * per_table, dtype=torch.int32, device="cuda")
@@ -182,7 +182,6 @@ def main():
     ap.add_argument("--learned-p", type=int, default=2621440, help="p of the learned-vs-plain comparison")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--batch-waves", type=int, default=4)
     ap.add_argument("--batch-learned", action="store_true",
                     help="also the 16,384-table FP32 Zipf batch with the learned index (2.4e9 items)")
     ap.add_argument("--cpu-n", type=int, default=10_000_000)

@@ -59,7 +59,7 @@ typedef struct {
     int32_t split_method;    /* atg_split_method */
     int32_t learned_segments;/* knots of the learned split index (default 4096) */
     int32_t record_phases;   /* 1: hipEvents around sum/partition/prefixsum/split/pack */
-    int32_t batch_waves;     /* batched build: waves per table workgroup (1..16); 0 = 4 */
+    int32_t reserved;        /* 0 */
 } atg_build_opts;
 
 typedef struct {

@@ -1,6 +1,6 @@
-"""Batched build (one persistent launch, one workgroup per table) against the
-oracle and against single-table builds of the same weights: bit-identical
-rows and W for every table; batch sampling against the oracle stream."""
+"""Batched build (one launch per phase over all tables) against the oracle and
+against single-table builds of the same weights: bit-identical rows and W
+for every table; batch sampling against the oracle stream."""
 import numpy as np
 import pytest
 
@@ -24,15 +24,15 @@ def make_batch(sizes, dist="exponential", seed=11, dtype=np.float64):
 SIZES = [1, 2, 3, 10, 16, 17, 100, 999, 1000, 4097, 10007, 65536, 100003, 31, 12345, 2, 250000]
 
 
-@pytest.mark.parametrize("dist,psa,split,waves", [
-    ("exponential", True, "bisect", 4),
-    ("pareto", True, "learned", 4),
-    ("unit_uniform", False, "bisect", 2),
-    ("heavy_mix", True, "bisect", 8),
+@pytest.mark.parametrize("dist,psa,split", [
+    ("exponential", True, "bisect"),
+    ("pareto", True, "learned"),
+    ("unit_uniform", False, "bisect"),
+    ("heavy_mix", True, "bisect"),
 ])
-def test_batch_matches_oracle_and_single(gpu, oracle, dist, psa, split, waves):
+def test_batch_matches_oracle_and_single(gpu, oracle, dist, psa, split):
     offsets, w = make_batch(SIZES, dist)
-    b = atg.Batch(offsets, psa_plus=psa, split=split, waves=waves)
+    b = atg.Batch(offsets, psa_plus=psa, split=split)
     b.set_weights(w)
     b.build()
     b.sync()
@@ -89,9 +89,9 @@ def test_batch_sampling_matches_oracle(gpu, oracle):
         np.testing.assert_array_equal(got[t * per:(t + 1) * per], exp)
 
 
-def test_batch_graph_replays_identical(gpu):
-    """The captured graph replayed several times, with other device work and
-    synchronisation between the replays, rebuilds the same tables."""
+def test_batch_rebuilds_identical(gpu):
+    """Repeated builds, with other device work and synchronisation between
+    them, rebuild the same tables."""
     torch = gpu
     offsets, w = make_batch([5000, 123, 40000, 64], "exponential")
     b = atg.Batch(offsets)
