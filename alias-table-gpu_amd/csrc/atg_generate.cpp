@@ -137,8 +137,11 @@ extern "C" atg_status atg_generate_weights(int32_t dist, int64_t n, uint64_t see
             });
             double sl = 0, sh = 0;
             for (int64_t i = 0; i < n; ++i) (heavy[i] ? sh : sl) += w[i];
-            const double s = (frac / (1.0 - frac)) * sl / sh;
-            parallel_for(n, [=, &heavy](int64_t i) { if (heavy[i]) w[i] *= s; });
+            // (No light items, as for n = 1: the heavy weights stay as drawn.)
+            if (sl > 0 && sh > 0) {
+                const double s = (frac / (1.0 - frac)) * sl / sh;
+                parallel_for(n, [=, &heavy](int64_t i) { if (heavy[i]) w[i] *= s; });
+            }
             break;
         }
         case ATG_DIST_ZIPF_SHUFFLED: {
