@@ -254,7 +254,7 @@ def main():
                                "then 1e9 sectioned samples (2^12-row sections) + rejection variant",
                    "n": n, "section_size": st["section_len"], "chains": args.chains, "p": p, "layout": "aos",
                    "psa_plus": True},
-        "roofline": {"bound": "hbm", "kernel": {"partition": "psa_wave", "sum": "sum_sections",
+        "roofline": {"bound": "hbm", "kernel": {"partition": "partition_local+partition_place", "sum": "sum_sections",
                                                 "pack": "pack_staged"}[dom],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

@@ -6,10 +6,10 @@
  * It restates, sequentially and in plain C++, the reference algorithms of
  * ByteHamster/alias-table-gpu under the canonical determinism contract that the
  * GPU kernels implement (fixed summation order, stable partitions, chained
- * section prefixes, bounded bisection, Philox4x32-10 streams).  Every function
+ * dyadic section prefixes, bounded bisection, Philox4x32-10 streams).  Every function
  * cites the reference file:line it follows.
  *
- * Parity pins (each function is checked by tests/test_oracle_*.py against an
+ * Parity pins (each function is checked by tests/test_oracle.py against an
  * independent fact, not against a retyped copy of itself):
  *   philox          Random123 known-answer vectors + rocRAND's open-source
  *                   philox4x32_10 engine (tests/golden/philox_xcheck.cpp)
