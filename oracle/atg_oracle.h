@@ -5,8 +5,8 @@
  *
  * It restates, sequentially and in plain C++, the reference algorithms of
  * ByteHamster/alias-table-gpu under the canonical determinism contract that the
- * GPU kernels implement (fixed summation order, stable partitions, chained
- * dyadic section prefixes, bounded bisection, Philox4x32-10 streams).  Every function
+ * GPU kernels implement (fixed summation order, stable partitions, dyadic
+ * section prefixes, bounded bisection, Philox4x32-10 streams).  Every function
  * cites the reference file:line it follows.
  *
  * Parity pins (each function is checked by tests/test_oracle.py against an
