@@ -94,7 +94,7 @@ __global__ __launch_bounds__(64) void seg_local(SegArgs g, SegMap m) {
     seg_locate(m, t, blk);
     const BuildArgs& a = g.tables[t];
     if (a.st->error) return;
-    psa_local<T, true, true>(a, blk, lds_raw, (int)threadIdx.x);
+    psa_local<T, true>(a, blk, lds_raw, (int)threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void seg_place(SegArgs g, SegMap m) {
@@ -104,8 +104,7 @@ __global__ __launch_bounds__(256) void seg_place(SegArgs g, SegMap m) {
     if (a.st->error) return;
     const int64_t s = blk * 4 + (threadIdx.x >> 6);
     if (s >= a.G) return;
-    double unused_l = 0.0, unused_h = 0.0;
-    place_section<false>(a, s, (int)(threadIdx.x & 63), unused_l, unused_h);
+    place_section(a, s, (int)(threadIdx.x & 63));
 }
 
 __global__ __launch_bounds__(256) void seg_degenerate(SegArgs g, SegMap m) {

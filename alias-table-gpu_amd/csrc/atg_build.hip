@@ -39,7 +39,7 @@ template <typename T, bool AOS>
 __global__ __launch_bounds__(64) void partition_local(BuildArgs a) {
     extern __shared__ double lds_raw[];
     if (a.st->error) return;
-    psa_local<T, AOS, true>(a, (int64_t)blockIdx.x, lds_raw, (int)threadIdx.x);
+    psa_local<T, AOS>(a, (int64_t)blockIdx.x, lds_raw, (int)threadIdx.x);
 }
 
 // Four sections per block, one per wave.
@@ -47,8 +47,7 @@ __global__ __launch_bounds__(256) void partition_place(BuildArgs a) {
     if (a.st->error) return;
     const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (s >= a.G) return;
-    double unused_l = 0.0, unused_h = 0.0;
-    place_section<false>(a, s, (int)(threadIdx.x & 63), unused_l, unused_h);
+    place_section(a, s, (int)(threadIdx.x & 63));
 }
 
 // ---------------------------------------------------------------------------

@@ -1,7 +1,7 @@
 // Device building blocks of the alias-table construction, shared by the
-// single-table kernels (one launch per phase) and the persistent batched
-// kernel (one workgroup per table).  Each function computes exactly the
-// canonical association, so both paths produce identical tables.
+// single-table kernels and the batched kernels (one launch per phase over all
+// tables).  Each function computes exactly the canonical association, so both
+// paths produce identical tables.
 #pragma once
 
 #include "atg_build.h"
@@ -47,6 +47,9 @@ __device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t
 
 
 #define PSA_MAX_LEN 2048
+// Chunks of 64 leftovers a wave keeps in flight at once when staging or
+// placing them.
+#define PLACE_GROUP 8
 
 struct SecLDS {
     double* w;      // [Ls]: lights from the front, heavies from the back
@@ -155,34 +158,10 @@ __device__ Prefix block_prefix(const BuildArgs& a, int64_t s, int lane, bool& ti
 // Partition pass 1, section s by one wave: stable light/heavy partition into
 // LDS, PSA+ greedy chains writing finished rows, the leftover order, the
 // canonical local-scan aggregates (published as the level-0 block of the
-// dyadic section tree) and the leftovers staged in rank order at the section's
-// own range (lights, then heavies).  Nothing here waits for another section.
-// PUBLISH: the block sums are completed by the sections themselves (agent-
-// scope slots, one section per workgroup); otherwise the level-0 sums are
-// plain stores and the caller builds the upper levels after a barrier.
-// P(s) from completed block sums with plain loads (same fold as block_prefix).
-__device__ __forceinline__ Prefix block_prefix_plain(const BuildArgs& a, int64_t s, int lane) {
-    uint64_t c = 0, l = d2u(0.0), h = d2u(0.0);
-    const bool set = lane < 62 && ((s >> lane) & 1);
-    if (set) {
-        const int64_t pos = (s >> (lane + 1)) << (lane + 1);
-        const int64_t idx = level_offset(a.G, lane) + (pos >> lane);
-        c = a.slots[idx];
-        l = a.slots[a.tree_total + idx];
-        h = a.slots[2 * a.tree_total + idx];
-    }
-    const uint64_t bits = __ballot(set);
-    Prefix P{0, 0.0, 0.0};
-    for (int b = 63; b >= 0; --b) {
-        if (!((bits >> b) & 1)) continue;
-        P.c += readlane_u64(c, b);
-        P.l = P.l + u2d(readlane_u64(l, b));
-        P.h = P.h + u2d(readlane_u64(h, b));
-    }
-    return P;
-}
-
-template <typename T, bool AOS, bool PUBLISH>
+// dyadic section tree; the second child of a pair to arrive completes the
+// parent) and the leftovers staged in rank order at the section's own range
+// (lights, then heavies).  Nothing here waits for another section's work.
+template <typename T, bool AOS>
 __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double* lds_raw, int lane) {
     DevState* st = a.st;
     const T* w = (const T*)a.weights;
@@ -240,28 +219,52 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
             jLo = t * stepsH; jHi = (t == C - 1) ? nH : (t + 1) * stepsH;
         }
         if (t < C && iHi - iLo > 0 && jHi - jLo > 0) {
+            // Registers hold the current light (lw, li), the next light
+            // (lw1, li1), the next two heavies (hw1/hi1, hw2/hi2); the load a
+            // step issues is first needed a step later, so LDS latency stays
+            // off the chain's critical path.  Prefetch indices are clamped to
+            // the chain's range; a clamped value is never consumed (the loop
+            // leaves before).
             int i = iLo, j = jLo;
             double wc = L.Hw(j);
             int hj = L.Hi(j);
             double lw = L.Lw(i);
             int li = L.Li(i);
+            int q = min(i + 1, iHi - 1);
+            double lw1 = L.Lw(q);
+            int li1 = L.Li(q);
+            q = min(j + 1, jHi - 1);
+            double hw1 = L.Hw(q);
+            int hi1 = L.Hi(q);
+            q = min(j + 2, jHi - 1);
+            double hw2 = L.Hw(q);
+            int hi2 = L.Hi(q);
             const double thr = W_N + ATG_EPSILON;
             while (true) {
                 const bool ex = wc <= thr;
                 if ((ex && j >= jHi - 1) || (!ex && i >= iHi - 1)) break;
-                const double hw1 = L.Hw(j + 1);
-                const int hi1 = L.Hi(j + 1);
                 const int item = ex ? hj : li;
                 const double rw = ex ? wc : lw;
                 const int alias = ex ? hi1 : hj;
                 const double next = ex ? hw1 : lw;
                 wc += next - W_N;
                 store_row<AOS>(a, start + item, rw, (int32_t)(start + alias));
-                j += ex ? 1 : 0;
-                i += ex ? 0 : 1;
-                hj = ex ? hi1 : hj;
-                lw = L.Lw(i);
-                li = L.Li(i);
+                if (ex) {
+                    ++j;
+                    hj = hi1;
+                    hw1 = hw2;
+                    hi1 = hi2;
+                    q = min(j + 2, jHi - 1);
+                    hw2 = L.Hw(q);
+                    hi2 = L.Hi(q);
+                } else {
+                    ++i;
+                    lw = lw1;
+                    li = li1;
+                    q = min(i + 1, iHi - 1);
+                    lw1 = L.Lw(q);
+                    li1 = L.Li(q);
+                }
             }
             // Remainder (PsaPlusNormal.cu:112-123): back to L when its
             // residual is light and a consumed light slot can hold it,
@@ -291,61 +294,39 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
     }
     wave_lds_sync();
 
-    // Canonical local scans (chunks of 64, wave scan, carries chained): aggregates.
-    double aggL = 0.0, aggH = 0.0;
-    for (int r0 = 0; r0 < mL; r0 += 64) {
-        const int r = r0 + lane;
-        const double v = (r < mL) ? L.Lw(identity ? r : (int)L.map[r]) : 0.0;
-        aggL = aggL + readlane_d(wave_scan(v), 63);
+    // One pass per list: stage the leftovers (weight, item) in rank order
+    // (lights at [start, start + mL), heavies at [start + mL, start + mL + mH))
+    // and form the canonical local-scan aggregate (chunks of 64, wave scan,
+    // chunk sums added left to right), PLACE_GROUP chunks at a time.
+    double agg[2] = {0.0, 0.0};
+    for (int hv = 0; hv < 2; ++hv) {
+        const bool heavy = hv == 1;
+        const int m = heavy ? mH : mL;
+        const int rank0 = heavy ? mL : 0;
+        for (int g0 = 0; g0 < m; g0 += 64 * PLACE_GROUP) {
+            double v[PLACE_GROUP];
+#pragma unroll
+            for (int u = 0; u < PLACE_GROUP; ++u) {
+                const int r = g0 + 64 * u + lane;
+                v[u] = 0.0;
+                if (r < m) {
+                    const int pos = identity ? r : (int)L.map[rank0 + r];
+                    v[u] = heavy ? L.Hw(pos) : L.Lw(pos);
+                    const int it = heavy ? L.Hi(pos) : L.Li(pos);
+                    a.stage_w[start + rank0 + r] = v[u];
+                    a.stage_i[start + rank0 + r] = (int32_t)(start + it);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PLACE_GROUP; ++u)
+                if (g0 + 64 * u < m) agg[hv] = agg[hv] + readlane_d(wave_scan(v[u]), 63);
+        }
     }
-    for (int r0 = 0; r0 < mH; r0 += 64) {
-        const int r = r0 + lane;
-        const double v = (r < mH) ? L.Hw(identity ? r : (int)L.map[mL + r]) : 0.0;
-        aggH = aggH + readlane_d(wave_scan(v), 63);
-    }
+    const double aggL = agg[0], aggH = agg[1];
     const uint64_t aggC = (uint64_t)(uint32_t)mL | ((uint64_t)(uint32_t)mH << 32);
-    // Stage the leftovers (weight, item) in rank order: lights at
-    // [start, start + mL), heavies at [start + mL, start + mL + mH).
-    for (int r0 = 0; r0 < mL + mH; r0 += 64) {
-        const int r = r0 + lane;
-        if (r < mL + mH) {
-            const bool heavy = r >= mL;
-            const int q = heavy ? r - mL : r;
-            const int pos = identity ? q : (int)L.map[r];
-            a.stage_w[start + r] = heavy ? L.Hw(pos) : L.Lw(pos);
-            a.stage_i[start + r] = (int32_t)(start + (heavy ? L.Hi(pos) : L.Li(pos)));
-        }
-    }
-    if (PUBLISH) {
-        bool timeout = false;
-        if (lane == 0) block_publish(a, s, aggC, aggL, aggH, timeout);
-        if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
-    } else if (lane == 0) {
-        a.slots[s] = aggC;
-        a.slots[a.tree_total + s] = d2u(aggL);
-        a.slots[2 * a.tree_total + s] = d2u(aggH);
-    }
-}
-
-// Upper levels of the dyadic block sums of one table by a whole workgroup,
-// level by level with barriers (the sums block_publish forms, same operands
-// and order: left child + right child).
-__device__ __forceinline__ void wg_block_tree(const BuildArgs& a, int tid, int nthreads) {
-    uint64_t* V = a.slots;
-    const int64_t G = a.G, tt = a.tree_total;
-    int64_t off_child = 0, off_parent = G;
-    for (int b = 1; ((int64_t)1 << b) <= G; ++b) {
-        const int64_t nb = G >> b;
-        for (int64_t c = tid; c < nb; c += nthreads) {
-            const int64_t l = off_child + 2 * c, r = l + 1;
-            V[off_parent + c] = V[l] + V[r];
-            V[tt + off_parent + c] = d2u(u2d(V[tt + l]) + u2d(V[tt + r]));
-            V[2 * tt + off_parent + c] = d2u(u2d(V[2 * tt + l]) + u2d(V[2 * tt + r]));
-        }
-        __syncthreads();
-        off_child = off_parent;
-        off_parent += nb;
-    }
+    bool timeout = false;
+    if (lane == 0) block_publish(a, s, aggC, aggL, aggH, timeout);
+    if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
 }
 
 // Partition pass 2, section s by one wave: the section prefix P(s) from the
@@ -353,24 +334,14 @@ __device__ __forceinline__ void wg_block_tree(const BuildArgs& a, int tid, int n
 // P(s) + local (the canonical chunked scan recomputed in the same order),
 // the split certificate, and for the last section the totals, sentinels and
 // split inputs.
-// BATCH: the block sums were completed before a workgroup barrier (plain
-// loads, no waiting) and the certificate extremes are folded into ext_l /
-// ext_h for the caller instead of the table state.
-template <bool BATCH>
-__device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int lane, double& ext_l_out,
-                                              double& ext_h_out) {
+__device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int lane) {
     DevState* st = a.st;
     const int64_t start = min(a.n, s * a.Ls);
     bool timeout = false;
-    uint64_t c0;
-    if (BATCH) {
-        c0 = a.slots[s];
-    } else {
-        uint64_t l0, h0;
-        block_load(a, s, c0, l0, h0, timeout);  // this section's own counts
-    }
+    uint64_t c0, l0, h0;
+    block_load(a, s, c0, l0, h0, timeout);  // this section's own counts
     const int mL = (int)(uint32_t)c0, mH = (int)(c0 >> 32);
-    const Prefix P = BATCH ? block_prefix_plain(a, s, lane) : block_prefix(a, s, lane, timeout);
+    const Prefix P = block_prefix(a, s, lane, timeout);
     const int64_t goffL = (int64_t)(uint32_t)P.c, goffH = (int64_t)(P.c >> 32);
     double ext_l = 0.0, ext_h = __builtin_inf();
     for (int hv = 0; hv < 2; ++hv) {
@@ -382,29 +353,45 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
         double* ow = heavy ? a.Hw : a.Lw;
         double* op = heavy ? a.Hp : a.Lp;
         const int64_t goff = heavy ? goffH : goffL;
+        // Groups of PLACE_GROUP chunks of 64: all loads of a group in flight
+        // together, the chunk scans independent, then the carries chained left
+        // to right (the same operations in the same order as one chunk at a
+        // time).
         double carry = 0.0;
-        for (int r0 = 0; r0 < m; r0 += 64) {
-            const int r = r0 + lane;
-            double v = 0.0;
-            int32_t item = 0;
-            if (r < m) {
-                v = a.stage_w[src + r];
-                item = a.stage_i[src + r];
+        for (int g0 = 0; g0 < m; g0 += 64 * PLACE_GROUP) {
+            double v[PLACE_GROUP], inc[PLACE_GROUP];
+            int32_t item[PLACE_GROUP];
+#pragma unroll
+            for (int u = 0; u < PLACE_GROUP; ++u) {
+                const int r = g0 + 64 * u + lane;
+                v[u] = 0.0;
+                item[u] = 0;
+                if (r < m) {
+                    v[u] = a.stage_w[src + r];
+                    item[u] = a.stage_i[src + r];
+                }
             }
-            const double inc = wave_scan(v);
-            const double local = carry + wave_shr1(inc);
-            const double pref = Pv + local;
-            const double carry_next = carry + readlane_d(inc, 63);
-            double succ = __shfl_down(pref, 1, 64);
-            if (lane == 63 || r == m - 1) succ = Pv + carry_next;
-            if (r < m) {
-                oi[goff + r] = item;
-                ow[goff + r] = v;
-                op[goff + r] = pref;
-                const double d = succ - pref;
-                if (heavy) ext_h = fmin(ext_h, d); else ext_l = fmax(ext_l, d);
+#pragma unroll
+            for (int u = 0; u < PLACE_GROUP; ++u) inc[u] = wave_scan(v[u]);
+#pragma unroll
+            for (int u = 0; u < PLACE_GROUP; ++u) {
+                const int r0 = g0 + 64 * u;
+                if (r0 >= m) break;
+                const int r = r0 + lane;
+                const double local = carry + wave_shr1(inc[u]);
+                const double pref = Pv + local;
+                const double carry_next = carry + readlane_d(inc[u], 63);
+                double succ = __shfl_down(pref, 1, 64);
+                if (lane == 63 || r == m - 1) succ = Pv + carry_next;
+                if (r < m) {
+                    oi[goff + r] = item[u];
+                    ow[goff + r] = v[u];
+                    op[goff + r] = pref;
+                    const double d = succ - pref;
+                    if (heavy) ext_h = fmin(ext_h, d); else ext_l = fmax(ext_l, d);
+                }
+                carry = carry_next;
             }
-            carry = carry_next;
         }
     }
 #pragma unroll
@@ -412,16 +399,13 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
         ext_l = fmax(ext_l, shfl_xor_d(ext_l, m));
         ext_h = fmin(ext_h, shfl_xor_d(ext_h, m));
     }
-    if (BATCH) {
-        if (mL > 0) ext_l_out = fmax(ext_l_out, ext_l);
-        if (mH > 0) ext_h_out = fmin(ext_h_out, ext_h);
-    } else if (lane == 0) {
+    if (lane == 0) {
         if (mL > 0) atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)d2u(ext_l));
         if (mH > 0) atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)d2u(ext_h));
     }
     if (s == a.G - 1) {
         // Totals P(G), sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
-        const Prefix Tot = BATCH ? block_prefix_plain(a, a.G, lane) : block_prefix(a, a.G, lane, timeout);
+        const Prefix Tot = block_prefix(a, a.G, lane, timeout);
         if (lane == 0) {
             const int64_t nLr = (int64_t)(uint32_t)Tot.c, nHr = (int64_t)(Tot.c >> 32);
             a.Li[nLr] = (int32_t)(a.n + 1); a.Lw[nLr] = 0.0; a.Lp[nLr] = Tot.l;
@@ -437,7 +421,7 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
             st->wToSplit = Tot.l + Tot.h;
         }
     }
-    if (!BATCH && __ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
+    if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
 }
 
 
