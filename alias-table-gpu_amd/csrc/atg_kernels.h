@@ -50,6 +50,9 @@ __device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t
 // Chunks of 64 leftovers a wave keeps in flight at once when staging or
 // placing them.
 #define PLACE_GROUP 8
+// Chunks of 64 leftovers the placement pass keeps in flight (a 2048-item
+// section has at most 32).
+#define PLACE_CHUNKS 16
 
 struct SecLDS {
     double* w;      // [Ls]: lights from the front, heavies from the back
@@ -155,6 +158,28 @@ __device__ Prefix block_prefix(const BuildArgs& a, int64_t s, int lane, bool& ti
     return P;
 }
 
+// P(s) from block sums completed by an earlier launch: plain loads, no
+// waiting (same fold as block_prefix).
+__device__ __forceinline__ Prefix block_prefix_done(const BuildArgs& a, int64_t s, int lane) {
+    uint64_t c = 0, l = d2u(0.0), h = d2u(0.0);
+    const bool set = lane < 62 && ((s >> lane) & 1);
+    if (set) {
+        const int64_t idx = level_offset(a.G, lane) + (((s >> (lane + 1)) << (lane + 1)) >> lane);
+        c = a.slots[idx];
+        l = a.slots[a.tree_total + idx];
+        h = a.slots[2 * a.tree_total + idx];
+    }
+    const uint64_t bits = __ballot(set);
+    Prefix P{0, 0.0, 0.0};
+    for (int b = 63; b >= 0; --b) {
+        if (!((bits >> b) & 1)) continue;
+        P.c += readlane_u64(c, b);
+        P.l = P.l + u2d(readlane_u64(l, b));
+        P.h = P.h + u2d(readlane_u64(h, b));
+    }
+    return P;
+}
+
 // Partition pass 1, section s by one wave: stable light/heavy partition into
 // LDS, PSA+ greedy chains writing finished rows, the leftover order, the
 // canonical local-scan aggregates (published as the level-0 block of the
@@ -225,6 +250,10 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
             // off the chain's critical path.  Prefetch indices are clamped to
             // the chain's range; a clamped value is never consumed (the loop
             // leaves before).
+            // Lights and heavies share one LDS array (heavy j at len-1-j), so
+            // the one load a step issues has a selected address; its value
+            // (nv, ni) is resolved into lw1 or hw2 only in the next step,
+            // after that step's decision, which needs lw and hw1 alone.
             int i = iLo, j = jLo;
             double wc = L.Hw(j);
             int hj = L.Hi(j);
@@ -239,6 +268,9 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
             q = min(j + 2, jHi - 1);
             double hw2 = L.Hw(q);
             int hi2 = L.Hi(q);
+            double nv = 0.0;
+            int ni = 0;
+            int pk = 0;  // pending load: 0 none, 1 -> (lw1, li1), 2 -> (hw2, hi2)
             const double thr = W_N + ATG_EPSILON;
             while (true) {
                 const bool ex = wc <= thr;
@@ -249,22 +281,23 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
                 const double next = ex ? hw1 : lw;
                 wc += next - W_N;
                 store_row<AOS>(a, start + item, rw, (int32_t)(start + alias));
-                if (ex) {
-                    ++j;
-                    hj = hi1;
-                    hw1 = hw2;
-                    hi1 = hi2;
-                    q = min(j + 2, jHi - 1);
-                    hw2 = L.Hw(q);
-                    hi2 = L.Hi(q);
-                } else {
-                    ++i;
-                    lw = lw1;
-                    li = li1;
-                    q = min(i + 1, iHi - 1);
-                    lw1 = L.Lw(q);
-                    li1 = L.Li(q);
-                }
+                // The previous step's load completes the prefetch registers.
+                lw1 = pk == 1 ? nv : lw1;
+                li1 = pk == 1 ? ni : li1;
+                hw2 = pk == 2 ? nv : hw2;
+                hi2 = pk == 2 ? ni : hi2;
+                // Advance one side and request the operand two steps ahead.
+                j += ex ? 1 : 0;
+                i += ex ? 0 : 1;
+                hj = ex ? hi1 : hj;
+                hw1 = ex ? hw2 : hw1;
+                hi1 = ex ? hi2 : hi1;
+                lw = ex ? lw : lw1;
+                li = ex ? li : li1;
+                const int pos = ex ? len - 1 - min(j + 2, jHi - 1) : min(i + 1, iHi - 1);
+                nv = L.w[pos];
+                ni = L.idx[pos];
+                pk = ex ? 2 : 1;
             }
             // Remainder (PsaPlusNormal.cu:112-123): back to L when its
             // residual is light and a consumed light slot can hold it,
@@ -337,11 +370,10 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
 __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int lane) {
     DevState* st = a.st;
     const int64_t start = min(a.n, s * a.Ls);
-    bool timeout = false;
-    uint64_t c0, l0, h0;
-    block_load(a, s, c0, l0, h0, timeout);  // this section's own counts
+    // The block sums were completed by the previous launch.
+    const uint64_t c0 = a.slots[s];  // this section's own counts
     const int mL = (int)(uint32_t)c0, mH = (int)(c0 >> 32);
-    const Prefix P = block_prefix(a, s, lane, timeout);
+    const Prefix P = block_prefix_done(a, s, lane);
     const int64_t goffL = (int64_t)(uint32_t)P.c, goffH = (int64_t)(P.c >> 32);
     double ext_l = 0.0, ext_h = __builtin_inf();
     for (int hv = 0; hv < 2; ++hv) {
@@ -353,16 +385,16 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
         double* ow = heavy ? a.Hw : a.Lw;
         double* op = heavy ? a.Hp : a.Lp;
         const int64_t goff = heavy ? goffH : goffL;
-        // Groups of PLACE_GROUP chunks of 64: all loads of a group in flight
+        // Groups of PLACE_CHUNKS chunks of 64: all loads of a group in flight
         // together, the chunk scans independent, then the carries chained left
         // to right (the same operations in the same order as one chunk at a
         // time).
         double carry = 0.0;
-        for (int g0 = 0; g0 < m; g0 += 64 * PLACE_GROUP) {
-            double v[PLACE_GROUP], inc[PLACE_GROUP];
-            int32_t item[PLACE_GROUP];
+        for (int g0 = 0; g0 < m; g0 += 64 * PLACE_CHUNKS) {
+            double v[PLACE_CHUNKS], inc[PLACE_CHUNKS];
+            int32_t item[PLACE_CHUNKS];
 #pragma unroll
-            for (int u = 0; u < PLACE_GROUP; ++u) {
+            for (int u = 0; u < PLACE_CHUNKS; ++u) {
                 const int r = g0 + 64 * u + lane;
                 v[u] = 0.0;
                 item[u] = 0;
@@ -372,25 +404,26 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
                 }
             }
 #pragma unroll
-            for (int u = 0; u < PLACE_GROUP; ++u) inc[u] = wave_scan(v[u]);
+            for (int u = 0; u < PLACE_CHUNKS; ++u) inc[u] = wave_scan(v[u]);
 #pragma unroll
-            for (int u = 0; u < PLACE_GROUP; ++u) {
+            for (int u = 0; u < PLACE_CHUNKS; ++u) {
                 const int r0 = g0 + 64 * u;
-                if (r0 >= m) break;
-                const int r = r0 + lane;
-                const double local = carry + wave_shr1(inc[u]);
-                const double pref = Pv + local;
-                const double carry_next = carry + readlane_d(inc[u], 63);
-                double succ = __shfl_down(pref, 1, 64);
-                if (lane == 63 || r == m - 1) succ = Pv + carry_next;
-                if (r < m) {
-                    oi[goff + r] = item[u];
-                    ow[goff + r] = v[u];
-                    op[goff + r] = pref;
-                    const double d = succ - pref;
-                    if (heavy) ext_h = fmin(ext_h, d); else ext_l = fmax(ext_l, d);
+                if (r0 < m) {
+                    const int r = r0 + lane;
+                    const double local = carry + wave_shr1(inc[u]);
+                    const double pref = Pv + local;
+                    const double carry_next = carry + readlane_d(inc[u], 63);
+                    double succ = __shfl_down(pref, 1, 64);
+                    if (lane == 63 || r == m - 1) succ = Pv + carry_next;
+                    if (r < m) {
+                        oi[goff + r] = item[u];
+                        ow[goff + r] = v[u];
+                        op[goff + r] = pref;
+                        const double d = succ - pref;
+                        if (heavy) ext_h = fmin(ext_h, d); else ext_l = fmax(ext_l, d);
+                    }
+                    carry = carry_next;
                 }
-                carry = carry_next;
             }
         }
     }
@@ -405,7 +438,7 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
     }
     if (s == a.G - 1) {
         // Totals P(G), sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
-        const Prefix Tot = block_prefix(a, a.G, lane, timeout);
+        const Prefix Tot = block_prefix_done(a, a.G, lane);
         if (lane == 0) {
             const int64_t nLr = (int64_t)(uint32_t)Tot.c, nHr = (int64_t)(Tot.c >> 32);
             a.Li[nLr] = (int32_t)(a.n + 1); a.Lw[nLr] = 0.0; a.Lp[nLr] = Tot.l;
@@ -421,7 +454,6 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
             st->wToSplit = Tot.l + Tot.h;
         }
     }
-    if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
 }
 
 
