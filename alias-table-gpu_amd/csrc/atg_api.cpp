@@ -324,7 +324,7 @@ atg_status atg_get_stats(atg_table* t, atg_stats* out) {
     double mdl, mdh;
     std::memcpy(&mdl, &st.cert_max_dL, 8);
     std::memcpy(&mdh, &st.cert_min_dH, 8);
-    out->monotone = mdl < mdh ? 1 : 0;
+    out->monotone = st.cert_done ? (mdl < mdh ? 1 : 0) : -1;
     if (t->events && t->built) {
         float ms;
         const int map[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}};

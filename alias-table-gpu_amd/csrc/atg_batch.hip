@@ -121,6 +121,19 @@ __global__ __launch_bounds__(256) void seg_split(SegArgs g, SegMap m) {
     split_bisect_block(g.tables[t], blk);
 }
 
+// Blocks of the certificate pass per table.
+__host__ __device__ inline int64_t cert_blocks(int64_t n) {
+    const int64_t b = (n + 2 + 2047) / 2048;
+    return b < 64 ? b : 64;
+}
+
+__global__ __launch_bounds__(256) void seg_cert(SegArgs g, SegMap m) {
+    int64_t t, blk;
+    seg_locate(m, t, blk);
+    const BuildArgs& a = g.tables[t];
+    cert_block(a, blk, cert_blocks(a.n));
+}
+
 __global__ __launch_bounds__(256) void seg_knots(SegArgs g, SegMap m) {
     int64_t t, blk;
     seg_locate(m, t, blk);
@@ -181,7 +194,7 @@ __global__ __launch_bounds__(256) void batch_sample(BatchSampleArgs a) {
 
 using namespace atg;
 
-enum { MAP_SUM, MAP_LOCAL, MAP_PLACE, MAP_DEG, MAP_SPLIT, MAP_KNOTS, MAP_PACK, MAP_COUNT };
+enum { MAP_SUM, MAP_LOCAL, MAP_PLACE, MAP_DEG, MAP_SPLIT, MAP_CERT, MAP_KNOTS, MAP_PACK, MAP_COUNT };
 
 struct atg_batch_s {
     int64_t ntables = 0, total = 0;
@@ -240,6 +253,7 @@ hipError_t launch_batch(const atg_batch* b, hipStream_t s) {
     hipLaunchKernelGGL(seg_place, grid(M[MAP_PLACE]), dim3(256), 0, s, g, M[MAP_PLACE]);
     hipLaunchKernelGGL(seg_degenerate, grid(M[MAP_DEG]), dim3(256), 0, s, g, M[MAP_DEG]);
     if (b->opts.split_method == ATG_SPLIT_LEARNED) {
+        hipLaunchKernelGGL(seg_cert, grid(M[MAP_CERT]), dim3(256), 0, s, g, M[MAP_CERT]);
         hipLaunchKernelGGL(seg_knots, grid(M[MAP_KNOTS]), dim3(256), 0, s, g, M[MAP_KNOTS]);
         hipLaunchKernelGGL(seg_learned, grid(M[MAP_SPLIT]), dim3(256), 0, s, g, M[MAP_SPLIT]);
     } else {
@@ -375,6 +389,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
             case MAP_PLACE: return (z.G + 3) / 4;
             case MAP_DEG: return degenerate_blocks(n);
             case MAP_SPLIT: return std::max<int64_t>(1, (z.p_req - 1 + 255) / 256);
+            case MAP_CERT: return cert_blocks(n);
             case MAP_KNOTS: return (K + 1 + 255) / 256;
             default: return (z.p_req + 63) / 64;  // MAP_PACK
         }

@@ -365,8 +365,7 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
 // Partition pass 2, section s by one wave: the section prefix P(s) from the
 // completed dyadic blocks, then the staged leftovers with prefixes
 // P(s) + local (the canonical chunked scan recomputed in the same order),
-// the split certificate, and for the last section the totals, sentinels and
-// split inputs.
+// and for the last section the totals, sentinels and split inputs.
 __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int lane) {
     DevState* st = a.st;
     const int64_t start = min(a.n, s * a.Ls);
@@ -375,7 +374,6 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
     const int mL = (int)(uint32_t)c0, mH = (int)(c0 >> 32);
     const Prefix P = block_prefix_done(a, s, lane);
     const int64_t goffL = (int64_t)(uint32_t)P.c, goffH = (int64_t)(P.c >> 32);
-    double ext_l = 0.0, ext_h = __builtin_inf();
     for (int hv = 0; hv < 2; ++hv) {
         const bool heavy = hv == 1;
         const int m = heavy ? mH : mL;
@@ -413,28 +411,15 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
                     const double local = carry + wave_shr1(inc[u]);
                     const double pref = Pv + local;
                     const double carry_next = carry + readlane_d(inc[u], 63);
-                    double succ = __shfl_down(pref, 1, 64);
-                    if (lane == 63 || r == m - 1) succ = Pv + carry_next;
                     if (r < m) {
                         oi[goff + r] = item[u];
                         ow[goff + r] = v[u];
                         op[goff + r] = pref;
-                        const double d = succ - pref;
-                        if (heavy) ext_h = fmin(ext_h, d); else ext_l = fmax(ext_l, d);
                     }
                     carry = carry_next;
                 }
             }
         }
-    }
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        ext_l = fmax(ext_l, shfl_xor_d(ext_l, m));
-        ext_h = fmin(ext_h, shfl_xor_d(ext_h, m));
-    }
-    if (lane == 0) {
-        if (mL > 0) atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)d2u(ext_l));
-        if (mH > 0) atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)d2u(ext_h));
     }
     if (s == a.G - 1) {
         // Totals P(G), sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
@@ -524,6 +509,7 @@ __device__ __forceinline__ void sum_block(const BuildArgs& a, int64_t blk, int64
         st->degenerate = 0;
         st->cert_max_dL = 0ull;                    // +0.0
         st->cert_min_dH = 0x7FF0000000000000ull;   // +inf
+        st->cert_done = 0;
         st->split_fallbacks = 0;
         st->learned_fallbacks = 0;
         st->sum_done = 0;
@@ -579,6 +565,43 @@ __device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, S
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Split certificate over the stored prefix sums (sentinels included): the
+// largest light increment Lp[t+1] - Lp[t] and the smallest heavy increment.
+// max dL < min dH makes the FOUND predicate monotone in j.  Block blk of nblk;
+// one atomic per block (max/min: the result is exact in any order).
+__device__ __forceinline__ void cert_block(const BuildArgs& a, int64_t blk, int64_t nblk) {
+    __shared__ uint64_t sh_l[16], sh_h[16];
+    DevState* st = a.st;
+    if (st->error) return;
+    const int64_t numL = st->numL, numH = st->numH;
+    double mx = 0.0, mn = __builtin_inf();
+    const int64_t stride = nblk * blockDim.x;
+    for (int64_t t = blk * blockDim.x + threadIdx.x; t + 1 < numL; t += stride) mx = fmax(mx, a.Lp[t + 1] - a.Lp[t]);
+    for (int64_t t = blk * blockDim.x + threadIdx.x; t + 1 < numH; t += stride) mn = fmin(mn, a.Hp[t + 1] - a.Hp[t]);
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        mx = fmax(mx, shfl_xor_d(mx, m));
+        mn = fmin(mn, shfl_xor_d(mn, m));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh_l[wave] = d2u(mx);
+        sh_h[wave] = d2u(mn);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t l = sh_l[0], h = sh_h[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            l = max(l, sh_l[q]);
+            h = min(h, sh_h[q]);
+        }
+        atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)l);
+        atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)h);
+        if (blk == 0) st->cert_done = 1u;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Learned split index helpers.

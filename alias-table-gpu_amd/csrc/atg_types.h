@@ -22,7 +22,7 @@ struct DevState {
     uint32_t error;
     uint32_t pad2;
     uint32_t degenerate;
-    uint32_t pad0;
+    uint32_t cert_done;       // the split certificate was computed (learned split)
     int64_t numL, numH;       // incl. sentinels
     int64_t p;                // effective splits
     int64_t nToSplit;
