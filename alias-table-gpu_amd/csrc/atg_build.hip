@@ -84,7 +84,8 @@ hipError_t launch_sum(const BuildArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-size_t psa_lds_bytes(int64_t Ls_max) { return (((size_t)Ls_max * 12) + 15) & ~(size_t)15; }
+// LDS of one partition wave: weights (8 B) and local item indices (2 B).
+size_t psa_lds_bytes(int64_t Ls_max) { return (((size_t)Ls_max * 10) + 15) & ~(size_t)15; }
 
 int64_t block_tree_total(int64_t G) {
     int64_t total = 0;

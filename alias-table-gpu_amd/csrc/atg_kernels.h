@@ -57,7 +57,6 @@ __device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t
 struct SecLDS {
     double* w;      // [Ls]: lights from the front, heavies from the back
     uint16_t* idx;  // [Ls]: local item index
-    uint16_t* map;  // [Ls]: leftover rank -> position (L ranks, then H ranks)
     int len;
     __device__ double& Lw(int i) { return w[i]; }
     __device__ uint16_t& Li(int i) { return idx[i]; }
@@ -180,6 +179,20 @@ __device__ __forceinline__ Prefix block_prefix_done(const BuildArgs& a, int64_t 
     return P;
 }
 
+// Position of leftover rank r of one class: chain t (lane t) holds its
+// leftovers at [lo_t, lo_t + count_t) and its ranks start at off_t (exclusive
+// scan of the counts, non-decreasing in t; lanes without a chain hold the
+// total).  The chain of r is the largest t with off_t <= r.  All lanes take part.
+__device__ __forceinline__ int chain_position(int r, int off, int lo) {
+    int t = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+        const int o = __shfl(off, t + step, 64);
+        if (o <= r) t += step;
+    }
+    return __shfl(lo, t, 64) + (r - __shfl(off, t, 64));
+}
+
 // Partition pass 1, section s by one wave: stable light/heavy partition into
 // LDS, PSA+ greedy chains writing finished rows, the leftover order, the
 // canonical local-scan aggregates (published as the level-0 block of the
@@ -196,7 +209,6 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
     SecLDS L;
     L.w = lds_raw;
     L.idx = reinterpret_cast<uint16_t*>(lds_raw + a.Ls_max);
-    L.map = L.idx + a.Ls_max;
     L.len = len;
     const uint64_t lt = lanemask_lt();
 
@@ -321,10 +333,6 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
     const int offL = wave_excl_scan_int(cl), offH = wave_excl_scan_int(ch);
     const int mL = wave_sum_int(cl), mH = wave_sum_int(ch);
     const bool identity = !a.psa_plus;
-    if (!identity) {
-        for (int q = 0; q < cl; ++q) L.map[offL + q] = (uint16_t)(iLo + q);
-        for (int q = 0; q < ch; ++q) L.map[mL + offH + q] = (uint16_t)(jLo + q);
-    }
     wave_lds_sync();
 
     // One pass per list: stage the leftovers (weight, item) in rank order
@@ -336,14 +344,15 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
         const bool heavy = hv == 1;
         const int m = heavy ? mH : mL;
         const int rank0 = heavy ? mL : 0;
+        const int off = heavy ? offH : offL, lo = heavy ? jLo : iLo;
         for (int g0 = 0; g0 < m; g0 += 64 * PLACE_GROUP) {
             double v[PLACE_GROUP];
 #pragma unroll
             for (int u = 0; u < PLACE_GROUP; ++u) {
                 const int r = g0 + 64 * u + lane;
+                const int pos = identity ? r : chain_position(r, off, lo);
                 v[u] = 0.0;
                 if (r < m) {
-                    const int pos = identity ? r : (int)L.map[rank0 + r];
                     v[u] = heavy ? L.Hw(pos) : L.Lw(pos);
                     const int it = heavy ? L.Hi(pos) : L.Li(pos);
                     a.stage_w[start + rank0 + r] = v[u];
