@@ -66,6 +66,7 @@ struct atg_table_s {
     double* d_section_sums = nullptr;
     double* d_tree = nullptr;
     uint64_t* d_slots = nullptr;
+    uint64_t* d_sec_cert = nullptr;
     double* d_stage_w = nullptr;
     int32_t* d_stage_i = nullptr;
     int32_t *d_Li = nullptr, *d_Hi = nullptr;
@@ -101,6 +102,7 @@ struct atg_table_s {
         a.tree_scratch = d_tree;
         a.tree_scratch_half = tree_half;
         a.slots = d_slots;
+        a.sec_cert = d_sec_cert;
         a.stage_w = d_stage_w;
         a.stage_i = d_stage_i;
         a.Li = d_Li; a.Lw = d_Lw; a.Lp = d_Lp;
@@ -141,7 +143,7 @@ int32_t atg_device_count(void) {
 
 static void free_table(atg_table* t) {
     if (!t) return;
-    void* ptrs[] = {t->d_weights_owned, t->d_state, t->d_section_sums, t->d_tree, t->d_slots, t->d_stage_w,
+    void* ptrs[] = {t->d_weights_owned, t->d_state, t->d_section_sums, t->d_tree, t->d_slots, t->d_sec_cert, t->d_stage_w,
                     t->d_stage_i, t->d_Li, t->d_Hi,
                     t->d_Lw, t->d_Lp, t->d_Hw, t->d_Hp, t->d_rows, t->d_tw, t->d_ta, t->d_splits, t->d_knots,
                     t->d_checksum, t->d_A, t->d_rej_tot, t->d_rej_off};
@@ -206,6 +208,7 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     ALLOC(t->d_Hw, n + 2);
     ALLOC(t->d_Hp, n + 2);
     ALLOC(t->d_stage_w, n);
+    ALLOC(t->d_sec_cert, 2 * t->G);
     ALLOC(t->d_stage_i, n);
     if (layout == ATG_AOS) {
         ALLOC(t->d_rows, n);

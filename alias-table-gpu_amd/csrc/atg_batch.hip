@@ -121,17 +121,11 @@ __global__ __launch_bounds__(256) void seg_split(SegArgs g, SegMap m) {
     split_bisect_block(g.tables[t], blk);
 }
 
-// Blocks of the certificate pass per table.
-__host__ __device__ inline int64_t cert_blocks(int64_t n) {
-    const int64_t b = (n + 2 + 2047) / 2048;
-    return b < 64 ? b : 64;
-}
-
-__global__ __launch_bounds__(256) void seg_cert(SegArgs g, SegMap m) {
-    int64_t t, blk;
-    seg_locate(m, t, blk);
-    const BuildArgs& a = g.tables[t];
-    cert_block(a, blk, cert_blocks(a.n));
+// The split certificate of every table: one block per table.
+__global__ __launch_bounds__(256) void seg_cert(SegArgs g) {
+    const BuildArgs& a = g.tables[blockIdx.x];
+    if (a.st->error) return;
+    cert_reduce_block(a);
 }
 
 __global__ __launch_bounds__(256) void seg_knots(SegArgs g, SegMap m) {
@@ -194,7 +188,7 @@ __global__ __launch_bounds__(256) void batch_sample(BatchSampleArgs a) {
 
 using namespace atg;
 
-enum { MAP_SUM, MAP_LOCAL, MAP_PLACE, MAP_DEG, MAP_SPLIT, MAP_CERT, MAP_KNOTS, MAP_PACK, MAP_COUNT };
+enum { MAP_SUM, MAP_LOCAL, MAP_PLACE, MAP_DEG, MAP_SPLIT, MAP_KNOTS, MAP_PACK, MAP_COUNT };
 
 struct atg_batch_s {
     int64_t ntables = 0, total = 0;
@@ -251,9 +245,9 @@ hipError_t launch_batch(const atg_batch* b, hipStream_t s) {
     hipLaunchKernelGGL(seg_sum<T>, grid(M[MAP_SUM]), dim3(1024), 0, s, g, M[MAP_SUM]);
     hipLaunchKernelGGL(seg_local<T>, grid(M[MAP_LOCAL]), dim3(64), psa_lds_bytes(b->Ls_max), s, g, M[MAP_LOCAL]);
     hipLaunchKernelGGL(seg_place, grid(M[MAP_PLACE]), dim3(256), 0, s, g, M[MAP_PLACE]);
+    hipLaunchKernelGGL(seg_cert, dim3((unsigned)b->ntables), dim3(256), 0, s, g);
     hipLaunchKernelGGL(seg_degenerate, grid(M[MAP_DEG]), dim3(256), 0, s, g, M[MAP_DEG]);
     if (b->opts.split_method == ATG_SPLIT_LEARNED) {
-        hipLaunchKernelGGL(seg_cert, grid(M[MAP_CERT]), dim3(256), 0, s, g, M[MAP_CERT]);
         hipLaunchKernelGGL(seg_knots, grid(M[MAP_KNOTS]), dim3(256), 0, s, g, M[MAP_KNOTS]);
         hipLaunchKernelGGL(seg_learned, grid(M[MAP_SPLIT]), dim3(256), 0, s, g, M[MAP_SPLIT]);
     } else {
@@ -320,7 +314,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         od[t] = nd;
         nd += align_up(z.G, 32) + align_up(2 * z.half, 32) + 5 * align_up(n + 2, 32);
         os[t] = ns;
-        ns += align_up(3 * z.tree_total + (z.tree_total + 1) / 2 + 1, 32);
+        ns += align_up(3 * z.tree_total + (z.tree_total + 1) / 2 + 1, 32) + align_up(2 * z.G, 32);
         oi[t] = ni;
         ni += 3 * align_up(n + 2, 64) + align_up(o.learned_segments + 2, 64);
         osp[t] = nsp;
@@ -365,6 +359,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         const int64_t cap = align_up(n + 2, 32);
         a.Lw = d; a.Lp = d + cap; a.Hw = d + 2 * cap; a.Hp = d + 3 * cap; a.stage_w = d + 4 * cap;
         a.slots = b->d_slots + os[t];
+        a.sec_cert = a.slots + align_up(3 * z.tree_total + (z.tree_total + 1) / 2 + 1, 32);
         int32_t* ip = b->d_ipool + oi[t];
         const int64_t icap = align_up(n + 2, 64);
         a.Li = ip; a.Hi = ip + icap; a.stage_i = ip + 2 * icap; a.knot_j = ip + 3 * icap;
@@ -389,7 +384,6 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
             case MAP_PLACE: return (z.G + 3) / 4;
             case MAP_DEG: return degenerate_blocks(n);
             case MAP_SPLIT: return std::max<int64_t>(1, (z.p_req - 1 + 255) / 256);
-            case MAP_CERT: return cert_blocks(n);
             case MAP_KNOTS: return (K + 1 + 255) / 256;
             default: return (z.p_req + 63) / 64;  // MAP_PACK
         }

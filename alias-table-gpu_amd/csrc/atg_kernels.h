@@ -383,15 +383,31 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
     const int mL = (int)(uint32_t)c0, mH = (int)(c0 >> 32);
     const Prefix P = block_prefix_done(a, s, lane);
     const int64_t goffL = (int64_t)(uint32_t)P.c, goffH = (int64_t)(P.c >> 32);
+    // Split certificate of this section: extreme increments between each
+    // stored prefix and its stored successor.  Inside the section the
+    // successor is the next lane / chunk; after the section's last item it is
+    // the first prefix of the next section holding that class, which is that
+    // section's P (local 0), or the class total P(G) of the sentinel.
+    double ext_l = 0.0, ext_h = __builtin_inf();
     for (int hv = 0; hv < 2; ++hv) {
         const bool heavy = hv == 1;
         const int m = heavy ? mH : mL;
+        if (m == 0) continue;
         const int64_t src = start + (heavy ? mL : 0);
         const double Pv = heavy ? P.h : P.l;
         int32_t* oi = heavy ? a.Hi : a.Li;
         double* ow = heavy ? a.Hw : a.Lw;
         double* op = heavy ? a.Hp : a.Lp;
         const int64_t goff = heavy ? goffH : goffL;
+        int64_t s2 = s + 1;
+        while (s2 < a.G) {
+            const uint64_t c2 = a.slots[s2];
+            if ((heavy ? (c2 >> 32) : (c2 & 0xFFFFFFFFull)) != 0) break;
+            ++s2;
+        }
+        const Prefix P2 = block_prefix_done(a, s2, lane);
+        const double succ_last = heavy ? P2.h : P2.l;
+        double ext = heavy ? __builtin_inf() : 0.0;
         // Groups of PLACE_CHUNKS chunks of 64: all loads of a group in flight
         // together, the chunk scans independent, then the carries chained left
         // to right (the same operations in the same order as one chunk at a
@@ -420,15 +436,31 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
                     const double local = carry + wave_shr1(inc[u]);
                     const double pref = Pv + local;
                     const double carry_next = carry + readlane_d(inc[u], 63);
+                    // The next chunk's first prefix is Pv + (carry_next + 0).
+                    double succ = __shfl_down(pref, 1, 64);
+                    if (lane == 63) succ = Pv + carry_next;
+                    if (r == m - 1) succ = succ_last;
                     if (r < m) {
                         oi[goff + r] = item[u];
                         ow[goff + r] = v[u];
                         op[goff + r] = pref;
+                        const double d = succ - pref;
+                        ext = heavy ? fmin(ext, d) : fmax(ext, d);
                     }
                     carry = carry_next;
                 }
             }
         }
+        if (heavy) ext_h = ext; else ext_l = ext;
+    }
+#pragma unroll
+    for (int q = 1; q < 64; q <<= 1) {
+        ext_l = fmax(ext_l, shfl_xor_d(ext_l, q));
+        ext_h = fmin(ext_h, shfl_xor_d(ext_h, q));
+    }
+    if (lane == 0) {
+        a.sec_cert[2 * s] = d2u(ext_l);
+        a.sec_cert[2 * s + 1] = d2u(ext_h);
     }
     if (s == a.G - 1) {
         // Totals P(G), sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
@@ -528,7 +560,7 @@ __device__ __forceinline__ void sum_block(const BuildArgs& a, int64_t blk, int64
 
 // ---------------------------------------------------------------------------
 // Pack helpers.
-#define PACK_CAP 2048
+#define PACK_CAP 1024
 
 struct GlobalLists {
     const int32_t *Li, *Hi;
@@ -576,39 +608,37 @@ __device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, S
 
 
 // ---------------------------------------------------------------------------
-// Split certificate over the stored prefix sums (sentinels included): the
-// largest light increment Lp[t+1] - Lp[t] and the smallest heavy increment.
-// max dL < min dH makes the FOUND predicate monotone in j.  Block blk of nblk;
-// one atomic per block (max/min: the result is exact in any order).
-__device__ __forceinline__ void cert_block(const BuildArgs& a, int64_t blk, int64_t nblk) {
+// Split certificate: max light increment and min heavy increment over all
+// stored prefix pairs (sentinels included), from the per-section extremes of
+// the placement pass.  max dL < min dH makes the FOUND predicate monotone in
+// j.  One block per table; max/min are exact in any order.
+__device__ __forceinline__ void cert_reduce_block(const BuildArgs& a) {
     __shared__ uint64_t sh_l[16], sh_h[16];
     DevState* st = a.st;
-    if (st->error) return;
-    const int64_t numL = st->numL, numH = st->numH;
-    double mx = 0.0, mn = __builtin_inf();
-    const int64_t stride = nblk * blockDim.x;
-    for (int64_t t = blk * blockDim.x + threadIdx.x; t + 1 < numL; t += stride) mx = fmax(mx, a.Lp[t + 1] - a.Lp[t]);
-    for (int64_t t = blk * blockDim.x + threadIdx.x; t + 1 < numH; t += stride) mn = fmin(mn, a.Hp[t + 1] - a.Hp[t]);
+    uint64_t l = 0ull, h = 0x7FF0000000000000ull;  // +0.0, +inf as ordered bits
+    for (int64_t s = threadIdx.x; s < a.G; s += blockDim.x) {
+        l = max(l, a.sec_cert[2 * s]);
+        h = min(h, a.sec_cert[2 * s + 1]);
+    }
 #pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        mx = fmax(mx, shfl_xor_d(mx, m));
-        mn = fmin(mn, shfl_xor_d(mn, m));
+    for (int q = 1; q < 64; q <<= 1) {
+        l = max(l, (uint64_t)__shfl_xor((unsigned long long)l, q, 64));
+        h = min(h, (uint64_t)__shfl_xor((unsigned long long)h, q, 64));
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) {
-        sh_l[wave] = d2u(mx);
-        sh_h[wave] = d2u(mn);
+        sh_l[wave] = l;
+        sh_h[wave] = h;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t l = sh_l[0], h = sh_h[0];
         for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
             l = max(l, sh_l[q]);
             h = min(h, sh_h[q]);
         }
-        atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)l);
-        atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)h);
-        if (blk == 0) st->cert_done = 1u;
+        st->cert_max_dL = l;
+        st->cert_min_dH = h;
+        st->cert_done = 1u;
     }
 }
 

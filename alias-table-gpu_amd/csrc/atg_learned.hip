@@ -7,22 +7,16 @@
 // model predicts j by linear interpolation between the knot values and the
 // search gallops outwards from the prediction inside [j_m, j_{m+1}].
 //
-// Bit identity with bisection: a certificate pass over the stored prefix sums
-// checks max dL < min dH (dL, dH: consecutive increments, sentinels included).  Then the
+// Bit identity with bisection: the partition certifies max dL < min dH over
+// the stored prefix sums (dL, dH: consecutive increments, sentinels included).  Then the
 // FOUND predicate of SplitMethodBasic (SplitMethodBasic.cuh:23-39) holds at
 // exactly one j and bisection returns it, so any exact search returns the same
 // (i, j, spill).  Without the certificate, or when the bracket check or the
 // local search fails, the k is resolved by full bisection (and counted).
-#include <algorithm>
-
 #include "atg_kernels.h"
 
 namespace atg {
 
-
-__global__ __launch_bounds__(256) void split_cert(BuildArgs a) {
-    cert_block(a, blockIdx.x, gridDim.x);
-}
 
 __global__ __launch_bounds__(256) void split_knots(BuildArgs a) {
     DevState* st = a.st;
@@ -47,8 +41,6 @@ hipError_t launch_split_learned(const BuildArgs& a, hipStream_t s) {
     if (e != hipSuccess) return e;
     const int64_t kmax = a.p_req;
     const int64_t K = a.learned_segments < kmax ? (int64_t)a.learned_segments : kmax;
-    const int64_t cert_blocks = std::min<int64_t>(1024, (a.n + 2 + 2047) / 2048);
-    hipLaunchKernelGGL(split_cert, dim3((unsigned)cert_blocks), dim3(256), 0, s, a);
     hipLaunchKernelGGL(split_knots, dim3((unsigned)((K + 1 + 255) / 256)), dim3(256), 0, s, a);
     if (kmax >= 2) hipLaunchKernelGGL(split_learned, dim3((unsigned)((kmax - 1 + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();

@@ -74,8 +74,8 @@ typedef struct {
     int64_t split_fallbacks; /* bisections that never reached FOUND (bounded; reported) */
     int64_t learned_fallbacks; /* learned-index predictions that fell back to bisection */
     int32_t degenerate;      /* one leftover class empty: every leftover row is {w, self} */
-    int32_t monotone;        /* split certificate max dL < min dH: 1 held, 0 not, -1 not computed
-                                (computed for the learned split only) */
+    int32_t monotone;        /* split certificate max dL < min dH over the stored prefix sums:
+                                1 held, 0 not, -1 not computed (no partition yet) */
     double phase_ms[5];      /* sum, partition(+PSA+ and prefix sums), prefixsum(0: fused), split, pack */
 } atg_stats;
 
