@@ -103,6 +103,7 @@ struct atg_table_s {
         a.tree_scratch_half = tree_half;
         a.slots = d_slots;
         a.sec_cert = d_sec_cert;
+        a.want_cert = opts.split_method == ATG_SPLIT_LEARNED;
         a.stage_w = d_stage_w;
         a.stage_i = d_stage_i;
         a.Li = d_Li; a.Lw = d_Lw; a.Lp = d_Lp;
@@ -208,7 +209,7 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     ALLOC(t->d_Hw, n + 2);
     ALLOC(t->d_Hp, n + 2);
     ALLOC(t->d_stage_w, n);
-    ALLOC(t->d_sec_cert, 2 * t->G);
+    ALLOC(t->d_sec_cert, 6 * t->G);
     ALLOC(t->d_stage_i, n);
     if (layout == ATG_AOS) {
         ALLOC(t->d_rows, n);

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void seg_split(SegArgs g, SegMap m) {
 __global__ __launch_bounds__(256) void seg_cert(SegArgs g) {
     const BuildArgs& a = g.tables[blockIdx.x];
     if (a.st->error) return;
-    cert_reduce_block(a);
+    cert_reduce_block(a, 0, 1);
 }
 
 __global__ __launch_bounds__(256) void seg_knots(SegArgs g, SegMap m) {
@@ -245,7 +245,8 @@ hipError_t launch_batch(const atg_batch* b, hipStream_t s) {
     hipLaunchKernelGGL(seg_sum<T>, grid(M[MAP_SUM]), dim3(1024), 0, s, g, M[MAP_SUM]);
     hipLaunchKernelGGL(seg_local<T>, grid(M[MAP_LOCAL]), dim3(64), psa_lds_bytes(b->Ls_max), s, g, M[MAP_LOCAL]);
     hipLaunchKernelGGL(seg_place, grid(M[MAP_PLACE]), dim3(256), 0, s, g, M[MAP_PLACE]);
-    hipLaunchKernelGGL(seg_cert, dim3((unsigned)b->ntables), dim3(256), 0, s, g);
+    if (b->opts.split_method == ATG_SPLIT_LEARNED)
+        hipLaunchKernelGGL(seg_cert, dim3((unsigned)b->ntables), dim3(256), 0, s, g);
     hipLaunchKernelGGL(seg_degenerate, grid(M[MAP_DEG]), dim3(256), 0, s, g, M[MAP_DEG]);
     if (b->opts.split_method == ATG_SPLIT_LEARNED) {
         hipLaunchKernelGGL(seg_knots, grid(M[MAP_KNOTS]), dim3(256), 0, s, g, M[MAP_KNOTS]);
@@ -314,7 +315,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         od[t] = nd;
         nd += align_up(z.G, 32) + align_up(2 * z.half, 32) + 5 * align_up(n + 2, 32);
         os[t] = ns;
-        ns += align_up(3 * z.tree_total + (z.tree_total + 1) / 2 + 1, 32) + align_up(2 * z.G, 32);
+        ns += align_up(3 * z.tree_total + (z.tree_total + 1) / 2 + 1, 32) + align_up(6 * z.G, 32);
         oi[t] = ni;
         ni += 3 * align_up(n + 2, 64) + align_up(o.learned_segments + 2, 64);
         osp[t] = nsp;
@@ -360,6 +361,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         a.Lw = d; a.Lp = d + cap; a.Hw = d + 2 * cap; a.Hp = d + 3 * cap; a.stage_w = d + 4 * cap;
         a.slots = b->d_slots + os[t];
         a.sec_cert = a.slots + align_up(3 * z.tree_total + (z.tree_total + 1) / 2 + 1, 32);
+        a.want_cert = o.split_method == ATG_SPLIT_LEARNED;
         int32_t* ip = b->d_ipool + oi[t];
         const int64_t icap = align_up(n + 2, 64);
         a.Li = ip; a.Hi = ip + icap; a.stage_i = ip + 2 * icap; a.knot_j = ip + 3 * icap;

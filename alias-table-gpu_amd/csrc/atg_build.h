@@ -24,7 +24,8 @@ struct BuildArgs {
     uint64_t* slots;          // dyadic block sums (counts, light, heavy) + arrival counters
     double* stage_w;          // n: leftovers staged by partition pass 1 (weights)
     int32_t* stage_i;         // n: leftovers staged by partition pass 1 (items)
-    uint64_t* sec_cert;       // 2 G: per-section split-certificate extremes (bit patterns)
+    uint64_t* sec_cert;       // 6 G: per-section split-certificate data (bit patterns)
+    int32_t want_cert;        // compute the split certificate (learned split)
     int32_t* Li; double* Lw; double* Lp;  // leftover lights (capacity n + 2)
     int32_t* Hi; double* Hw; double* Hp;  // leftover heavies (capacity n + 2)
     Row* rows;                // AoS table or null

@@ -42,10 +42,10 @@ __global__ __launch_bounds__(64) void partition_local(BuildArgs a) {
     psa_local<T, AOS>(a, (int64_t)blockIdx.x, lds_raw, (int)threadIdx.x);
 }
 
-// The split certificate from the per-section extremes (one block).
-__global__ __launch_bounds__(1024) void cert_reduce(BuildArgs a) {
+// The split certificate from the per-section data.
+__global__ __launch_bounds__(256) void cert_reduce(BuildArgs a) {
     if (a.st->error) return;
-    cert_reduce_block(a);
+    cert_reduce_block(a, blockIdx.x, gridDim.x);
 }
 
 // Four sections per block, one per wave.
@@ -110,7 +110,8 @@ hipError_t launch_partition(const BuildArgs& a, hipStream_t s) {
         else hipLaunchKernelGGL((partition_local<double, false>), g, b, lds, s, a);
     }
     hipLaunchKernelGGL(partition_place, dim3(blocks_for(a.G, 4)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(cert_reduce, dim3(1), dim3(1024), 0, s, a);
+    if (a.want_cert)
+        hipLaunchKernelGGL(cert_reduce, dim3(blocks_for(a.G, 256 * 16)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -383,11 +383,10 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
     const int mL = (int)(uint32_t)c0, mH = (int)(c0 >> 32);
     const Prefix P = block_prefix_done(a, s, lane);
     const int64_t goffL = (int64_t)(uint32_t)P.c, goffH = (int64_t)(P.c >> 32);
-    // Split certificate of this section: extreme increments between each
-    // stored prefix and its stored successor.  Inside the section the
-    // successor is the next lane / chunk; after the section's last item it is
-    // the first prefix of the next section holding that class, which is that
-    // section's P (local 0), or the class total P(G) of the sentinel.
+    // Split certificate (learned split only): the extreme increments between
+    // consecutive stored prefixes inside the section, and the section's first
+    // and last prefix of each class; the pairs across sections are closed by
+    // cert_reduce_block.
     double ext_l = 0.0, ext_h = __builtin_inf();
     for (int hv = 0; hv < 2; ++hv) {
         const bool heavy = hv == 1;
@@ -399,14 +398,6 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
         double* ow = heavy ? a.Hw : a.Lw;
         double* op = heavy ? a.Hp : a.Lp;
         const int64_t goff = heavy ? goffH : goffL;
-        int64_t s2 = s + 1;
-        while (s2 < a.G) {
-            const uint64_t c2 = a.slots[s2];
-            if ((heavy ? (c2 >> 32) : (c2 & 0xFFFFFFFFull)) != 0) break;
-            ++s2;
-        }
-        const Prefix P2 = block_prefix_done(a, s2, lane);
-        const double succ_last = heavy ? P2.h : P2.l;
         double ext = heavy ? __builtin_inf() : 0.0;
         // Groups of PLACE_CHUNKS chunks of 64: all loads of a group in flight
         // together, the chunk scans independent, then the carries chained left
@@ -436,16 +427,20 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
                     const double local = carry + wave_shr1(inc[u]);
                     const double pref = Pv + local;
                     const double carry_next = carry + readlane_d(inc[u], 63);
-                    // The next chunk's first prefix is Pv + (carry_next + 0).
-                    double succ = __shfl_down(pref, 1, 64);
-                    if (lane == 63) succ = Pv + carry_next;
-                    if (r == m - 1) succ = succ_last;
                     if (r < m) {
                         oi[goff + r] = item[u];
                         ow[goff + r] = v[u];
                         op[goff + r] = pref;
-                        const double d = succ - pref;
-                        ext = heavy ? fmin(ext, d) : fmax(ext, d);
+                    }
+                    if (a.want_cert) {
+                        // The next chunk's first prefix is Pv + (carry_next + 0).
+                        double succ = __shfl_down(pref, 1, 64);
+                        if (lane == 63) succ = Pv + carry_next;
+                        if (r < m - 1) {
+                            const double d = succ - pref;
+                            ext = heavy ? fmin(ext, d) : fmax(ext, d);
+                        }
+                        if (r == m - 1) a.sec_cert[6 * s + (heavy ? 5 : 3)] = d2u(pref);
                     }
                     carry = carry_next;
                 }
@@ -453,14 +448,19 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
         }
         if (heavy) ext_h = ext; else ext_l = ext;
     }
+    if (a.want_cert) {
 #pragma unroll
-    for (int q = 1; q < 64; q <<= 1) {
-        ext_l = fmax(ext_l, shfl_xor_d(ext_l, q));
-        ext_h = fmin(ext_h, shfl_xor_d(ext_h, q));
-    }
-    if (lane == 0) {
-        a.sec_cert[2 * s] = d2u(ext_l);
-        a.sec_cert[2 * s + 1] = d2u(ext_h);
+        for (int q = 1; q < 64; q <<= 1) {
+            ext_l = fmax(ext_l, shfl_xor_d(ext_l, q));
+            ext_h = fmin(ext_h, shfl_xor_d(ext_h, q));
+        }
+        if (lane == 0) {
+            // [0] max dL, [1] min dH inside; [2] first L, [3] last L, [4] first H, [5] last H.
+            a.sec_cert[6 * s] = d2u(ext_l);
+            a.sec_cert[6 * s + 1] = d2u(ext_h);
+            a.sec_cert[6 * s + 2] = d2u(P.l);
+            a.sec_cert[6 * s + 4] = d2u(P.h);
+        }
     }
     if (s == a.G - 1) {
         // Totals P(G), sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
@@ -560,7 +560,7 @@ __device__ __forceinline__ void sum_block(const BuildArgs& a, int64_t blk, int64
 
 // ---------------------------------------------------------------------------
 // Pack helpers.
-#define PACK_CAP 1024
+#define PACK_CAP 2048
 
 struct GlobalLists {
     const int32_t *Li, *Hi;
@@ -609,36 +609,52 @@ __device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, S
 
 // ---------------------------------------------------------------------------
 // Split certificate: max light increment and min heavy increment over all
-// stored prefix pairs (sentinels included), from the per-section extremes of
-// the placement pass.  max dL < min dH makes the FOUND predicate monotone in
-// j.  One block per table; max/min are exact in any order.
-__device__ __forceinline__ void cert_reduce_block(const BuildArgs& a) {
+// consecutive stored prefix pairs (sentinels included).  Per section s: the
+// extremes inside it, and the pair from its last prefix of a class to the
+// first prefix of the next section holding that class (that section's P) or
+// to the class total P(G) of the sentinel.  max dL < min dH makes the FOUND
+// predicate monotone in j.  Block blk of nblk folds sections blk, blk + nblk,
+// ... with atomics on the state (max/min are exact in any order).
+__device__ __forceinline__ void cert_reduce_block(const BuildArgs& a, int64_t blk, int64_t nblk) {
     __shared__ uint64_t sh_l[16], sh_h[16];
     DevState* st = a.st;
-    uint64_t l = 0ull, h = 0x7FF0000000000000ull;  // +0.0, +inf as ordered bits
-    for (int64_t s = threadIdx.x; s < a.G; s += blockDim.x) {
-        l = max(l, a.sec_cert[2 * s]);
-        h = min(h, a.sec_cert[2 * s + 1]);
+    const int64_t G = a.G;
+    const double totL = a.Lp[st->numL - 1], totH = a.Hp[st->numH - 1];
+    double l = 0.0, h = __builtin_inf();
+    for (int64_t s = blk * blockDim.x + threadIdx.x; s < G; s += nblk * blockDim.x) {
+        const uint64_t* c = a.sec_cert + 6 * s;
+        const uint64_t cnt = a.slots[s];
+        l = fmax(l, u2d(c[0]));
+        h = fmin(h, u2d(c[1]));
+        for (int hv = 0; hv < 2; ++hv) {
+            if (((hv ? cnt >> 32 : cnt & 0xFFFFFFFFull)) == 0) continue;
+            int64_t s2 = s + 1;
+            while (s2 < G && ((hv ? a.slots[s2] >> 32 : a.slots[s2] & 0xFFFFFFFFull)) == 0) ++s2;
+            const double next = s2 < G ? u2d(a.sec_cert[6 * s2 + (hv ? 4 : 2)]) : (hv ? totH : totL);
+            const double d = next - u2d(c[hv ? 5 : 3]);
+            if (hv) h = fmin(h, d); else l = fmax(l, d);
+        }
     }
 #pragma unroll
     for (int q = 1; q < 64; q <<= 1) {
-        l = max(l, (uint64_t)__shfl_xor((unsigned long long)l, q, 64));
-        h = min(h, (uint64_t)__shfl_xor((unsigned long long)h, q, 64));
+        l = fmax(l, shfl_xor_d(l, q));
+        h = fmin(h, shfl_xor_d(h, q));
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) {
-        sh_l[wave] = l;
-        sh_h[wave] = h;
+        sh_l[wave] = d2u(l);
+        sh_h[wave] = d2u(h);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        uint64_t bl = sh_l[0], bh = sh_h[0];
         for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
-            l = max(l, sh_l[q]);
-            h = min(h, sh_h[q]);
+            bl = max(bl, sh_l[q]);
+            bh = min(bh, sh_h[q]);
         }
-        st->cert_max_dL = l;
-        st->cert_min_dH = h;
-        st->cert_done = 1u;
+        atomicMax((unsigned long long*)&st->cert_max_dL, (unsigned long long)bl);
+        atomicMin((unsigned long long*)&st->cert_min_dH, (unsigned long long)bh);
+        if (blk == 0) st->cert_done = 1u;
     }
 }
 

@@ -75,7 +75,7 @@ typedef struct {
     int64_t learned_fallbacks; /* learned-index predictions that fell back to bisection */
     int32_t degenerate;      /* one leftover class empty: every leftover row is {w, self} */
     int32_t monotone;        /* split certificate max dL < min dH over the stored prefix sums:
-                                1 held, 0 not, -1 not computed (no partition yet) */
+                                1 held, 0 not, -1 not computed (only the learned split needs it) */
     double phase_ms[5];      /* sum, partition(+PSA+ and prefix sums), prefixsum(0: fused), split, pack */
 } atg_stats;
 
