@@ -152,6 +152,7 @@ __global__ __launch_bounds__(256) void sample_sectioned(SampleArgs a) {
 // scanning the totals, then each block writes its items' slots of A.
 #define REJ_ITEMS 4096
 #define REJ_THREADS 256
+#define REJ_SHORT 8  // runs longer than this are written by the whole block
 
 template <typename T>
 __device__ __forceinline__ uint32_t rej_count(const RejArgs& a, int64_t i, double W) {
@@ -213,10 +214,14 @@ template <typename T>
 __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
     __shared__ uint64_t incl[REJ_ITEMS];
     __shared__ uint64_t wsum[REJ_THREADS / 64];
+    __shared__ int32_t big[REJ_ITEMS];
+    __shared__ uint32_t nbig;
     if (a.st->error) return;
     const double W = a.st->W;
+    const T* w = (const T*)a.weights;
     const int64_t base = (int64_t)blockIdx.x * REJ_ITEMS;
     const int cnt = (int)min<int64_t>(REJ_ITEMS, a.n - base);
+    if (threadIdx.x == 0) nbig = 0;
     // Per-thread blocked items (16 consecutive), inclusive offsets in LDS.
     const int per = REJ_ITEMS / REJ_THREADS;
     uint64_t loc[REJ_ITEMS / REJ_THREADS];
@@ -240,29 +245,39 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
             if (k < cnt) a.incl_out[base + k] = incl[k];
         }
     __syncthreads();
-    // Slots [boff, incl[cnt-1]) of this block, written coalesced; the item of a
-    // slot is found by bisection over the block's inclusive offsets.  The first
-    // slot of a run carries the item's acceptance fraction (FP64; the
-    // reference computes it in float, RejectionSampling.cu:26-28).
-    const uint64_t s_end = incl[cnt - 1];
-    const T* w = (const T*)a.weights;
-    for (uint64_t s = boff + threadIdx.x; s < s_end; s += REJ_THREADS) {
-        int lo = 0, hi = cnt - 1;  // first k with incl[k] > s
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (incl[mid] > s) hi = mid; else lo = mid + 1;
-        }
-        const uint64_t first = lo == 0 ? boff : incl[lo - 1];
-        double accept = 1.0;
-        if (s == first) {
-            const double x = ((double)a.n * (double)w[base + lo]) / W;
-            accept = x - floor(x);
-        }
+    // Each item's run [incl[k-1], incl[k]) of A.  The first slot carries the
+    // item's acceptance fraction (FP64; the reference computes it in float,
+    // RejectionSampling.cu:26-28), the others accept always.  Runs of up to
+    // REJ_SHORT slots are written by one thread, items striped over the
+    // threads so that a wave's stores are nearly contiguous; longer runs are
+    // listed and written by the whole block.
+    for (int k = threadIdx.x; k < cnt; k += REJ_THREADS) {
+        const uint64_t first = k == 0 ? boff : incl[k - 1];
+        const uint64_t len = incl[k] - first;
+        const double x = ((double)a.n * (double)w[base + k]) / W;
         RejEntry e;
-        e.accept = accept;
-        e.item = (int32_t)(base + lo);
+        e.accept = x - floor(x);
+        e.item = (int32_t)(base + k);
         e.pad = 0;
-        *reinterpret_cast<uint4*>(&a.A[s]) = *reinterpret_cast<const uint4*>(&e);
+        *reinterpret_cast<uint4*>(&a.A[first]) = *reinterpret_cast<const uint4*>(&e);
+        if (len > REJ_SHORT) {
+            big[atomicAdd(&nbig, 1u)] = k;
+        } else {
+            e.accept = 1.0;
+            for (uint64_t q = 1; q < len; ++q) *reinterpret_cast<uint4*>(&a.A[first + q]) = *reinterpret_cast<const uint4*>(&e);
+        }
+    }
+    __syncthreads();
+    for (uint32_t b = 0; b < nbig; ++b) {
+        const int k = big[b];
+        const uint64_t first = k == 0 ? boff : incl[k - 1];
+        const uint64_t len = incl[k] - first;
+        RejEntry e;
+        e.accept = 1.0;
+        e.item = (int32_t)(base + k);
+        e.pad = 0;
+        for (uint64_t q = 1 + threadIdx.x; q < len; q += REJ_THREADS)
+            *reinterpret_cast<uint4*>(&a.A[first + q]) = *reinterpret_cast<const uint4*>(&e);
     }
 }
 

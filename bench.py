@@ -174,7 +174,7 @@ def main():
     ap.add_argument("--samples", type=int, default=1_000_000_000)
     ap.add_argument("--section-size", type=int, default=2048,
                     help="PSA+ section (0 = the reference's min(1000, N/10))")
-    ap.add_argument("--chains", type=int, default=16)
+    ap.add_argument("--chains", type=int, default=32)
     ap.add_argument("--num-splits", type=int, default=2621440, help="p (0 = the reference's AUTO, 320 per CU)")
     ap.add_argument("--split", choices=["bisect", "learned"], default="bisect")
     ap.add_argument("--learned-segments", type=int, default=4096)
