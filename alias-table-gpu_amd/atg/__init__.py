@@ -13,7 +13,7 @@ from ._lib import (ATG_AOS, ATG_F32, ATG_F64, ATG_MEM_DEVICE, ATG_MEM_HOST, ATG_
                    ATG_SAMPLE_SECTIONED, ATG_SOA, ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED, DIST, AtgError,
                    BuildOpts, Stats, check, lib)
 
-__all__ = ["Table", "Batch", "generate_weights", "generate_batch", "default_opts", "lib", "AtgError", "DIST"]
+__all__ = ["Table", "Batch", "autotune", "generate_weights", "generate_batch", "default_opts", "lib", "AtgError", "DIST"]
 
 
 def _ptr(a):
@@ -59,6 +59,28 @@ def generate_batch(dist, ntables, lo, hi, seed=42, param=0.0, dtype="f64"):
     check(lib().atg_generate_batch(d, int(ntables), int(lo), int(hi), int(seed), float(param), _ptr(offsets),
                                    _ptr(w), ATG_F32 if dtype == "f32" else ATG_F64))
     return offsets, w
+
+
+def autotune(w, dtype="f64", layout="aos", psa_plus=True, split="bisect", learned_segments=4096, reps=3):
+    """Build shape (section_size, chains, num_splits) chosen by atg_autotune
+    for these weights on this device, and the median build time in ms
+    (negative when it came from the ATG_TUNE_CACHE file)."""
+    o = default_opts()
+    o.psa_plus = 1 if psa_plus else 0
+    o.split_method = ATG_SPLIT_LEARNED if split == "learned" else ATG_SPLIT_BISECT
+    o.learned_segments = int(learned_segments)
+    best = BuildOpts()
+    ms = C.c_double(0.0)
+    if hasattr(w, "is_cuda") and w.is_cuda:
+        ptr, mem, n = _ptr(w), ATG_MEM_DEVICE, w.numel()
+    else:
+        w = np.ascontiguousarray(w, dtype=np.float32 if dtype == "f32" else np.float64)
+        ptr, mem, n = _ptr(w), ATG_MEM_HOST, w.size
+    check(lib().atg_autotune(ptr, mem, int(n), ATG_F32 if dtype == "f32" else ATG_F64,
+                             ATG_SOA if layout == "soa" else ATG_AOS, C.byref(o), int(reps), C.byref(best),
+                             C.byref(ms)))
+    return {"section_size": int(best.section_size), "chains": int(best.chains), "num_splits": int(best.num_splits),
+            "split": split, "psa_plus": psa_plus}, ms.value
 
 
 class Table:

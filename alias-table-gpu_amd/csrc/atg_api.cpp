@@ -58,6 +58,7 @@ struct atg_table_s {
     atg_layout layout = ATG_AOS;
     atg_build_opts opts{};
     int64_t G = 0, Ls = 0, tree_total = 0, p_req = 0, tree_half = 0;
+    int64_t p_cap = 0;  // capacity of the splits array minus one
     int device = 0;
     // device memory
     void* d_weights_owned = nullptr;
@@ -217,6 +218,7 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
         ALLOC(t->d_tw, n);
         ALLOC(t->d_ta, n);
     }
+    t->p_cap = t->p_req;
     ALLOC(t->d_splits, t->p_req + 1);
     ALLOC(t->d_knots, o.learned_segments + 2);
     ALLOC(t->d_checksum, 1);
@@ -592,3 +594,22 @@ atg_status atg_rejection_sample(atg_table* t, uint64_t count, uint64_t seed, int
 }
 
 }  // extern "C"
+
+namespace atg {
+
+// Changes the chain count and the requested splits of a created table without
+// reallocating (the autotuner's inner loop); num_splits is clamped as
+// atg_create clamps it and must fit the splits array.
+atg_status table_set_shape(atg_table* t, int32_t chains, int64_t num_splits) {
+    if (!t) return fail(ATG_EINVAL, "table_set_shape: NULL table");
+    if (chains < 1 || chains > 64) return fail(ATG_EINVAL, "table_set_shape: chains=%d", chains);
+    const int64_t p = std::max<int64_t>(2, std::min<int64_t>(num_splits, t->n / 10));
+    if (p > t->p_cap) return fail(ATG_EINVAL, "table_set_shape: %lld splits exceed %lld", (long long)p, (long long)t->p_cap);
+    t->opts.chains = chains;
+    t->opts.num_splits = num_splits;
+    t->p_req = p;
+    t->built = false;
+    return ATG_OK;
+}
+
+}  // namespace atg

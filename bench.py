@@ -87,31 +87,48 @@ def time_builds(t, stream, steps, warmup):
 def bench_learned(args, stream):
     """Learned split index vs bisection on N=1e8 weights where 1% of the items
     carry 90% of the mass (lognormal(10, 1) heavies rescaled, U(0,1) lights):
-    whole-build times at the same p, and bit identity of the two tables."""
+    whole-build times of each method at its own autotuned shape and of the
+    learned index at the plain build's shape, whose table must be bit-identical
+    to the plain table."""
     import numpy as np
     import atg
     n = args.n
     w = atg.generate_weights("heavy_mix", n, seed=42, param=0.9)
-    res = {"workload": f"N={n:.0e} heavy_mix (1% heavy, 90% mass), p={args.learned_p}, "
-                       f"{args.learned_segments} segments", "p": args.learned_p}
+    res = {"workload": f"N={n:.0e} heavy_mix (1% heavy, 90% mass), {args.learned_segments} segments; "
+                       f"each split method at its own autotuned shape"}
     tabs = {}
     for split in ("bisect", "learned"):
-        t = atg.Table(n, psa_plus=True, chains=args.chains, section_size=args.section_size,
-                      num_splits=args.learned_p, split=split, learned_segments=args.learned_segments,
+        if args.no_autotune:
+            shape = {"section_size": args.section_size, "chains": args.chains, "num_splits": args.learned_p}
+        else:
+            shape, _ = atg.autotune(w, split=split, learned_segments=args.learned_segments)
+        t = atg.Table(n, psa_plus=True, chains=shape["chains"], section_size=shape["section_size"],
+                      num_splits=shape["num_splits"], split=split, learned_segments=args.learned_segments,
                       record_phases=True)
         t.set_weights(w, stream)
         ms = time_builds(t, stream, args.steps, args.warmup)
         st = t.stats()
         res[split] = {"build_ms": round(ms, 4), "split_ms": round(st["phase_ms"][3], 4),
-                      "gitems_s": round(n / (ms * 1e-3) / 1e9, 3)}
+                      "gitems_s": round(n / (ms * 1e-3) / 1e9, 3), "section_size": shape["section_size"],
+                      "chains": shape["chains"], "p": st["p"]}
         if split == "learned":
             res[split]["learned_fallbacks"] = st["learned_fallbacks"]
             res[split]["monotone"] = st["monotone"]
-        tabs[split] = t.table()
+        if split == "bisect":
+            tabs[split] = t.table()
         del t
     res["speedup"] = round(res["bisect"]["build_ms"] / res["learned"]["build_ms"], 4)
     res["split_speedup"] = round(res["bisect"]["split_ms"] / max(res["learned"]["split_ms"], 1e-9), 3)
-    a, b = tabs["bisect"], tabs["learned"]
+    # Bit identity: the learned index at the plain build's shape must give the
+    # plain build's table.
+    sh = res["bisect"]
+    t = atg.Table(n, psa_plus=True, chains=sh["chains"], section_size=sh["section_size"], num_splits=sh["p"],
+                  split="learned", learned_segments=args.learned_segments, record_phases=True)
+    t.set_weights(w, stream)
+    res["learned_same_shape_build_ms"] = round(time_builds(t, stream, args.steps, args.warmup), 4)
+    res["speedup_same_shape"] = round(res["bisect"]["build_ms"] / res["learned_same_shape_build_ms"], 4)
+    a, b = tabs["bisect"], t.table()
+    del t
     res["bit_identical"] = bool(np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64)) and
                                 np.array_equal(a[1], b[1]))
     log(f"[bench] learned vs plain: {res}")
@@ -187,6 +204,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sampling", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --section-size/--chains/--num-splits instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard into "
@@ -203,6 +222,13 @@ def main():
     log(f"[bench] generating N={n} Pareto(1.1) weights")
     w = atg.generate_weights("pareto", n, seed=42, param=1.1)
     stream = torch.cuda.current_stream()
+    tuned = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        shape, tms = atg.autotune(w, split=args.split, learned_segments=args.learned_segments)
+        tuned = dict(shape, median_ms=round(tms, 4), tuning_s=round(time.perf_counter() - t0, 2))
+        args.section_size, args.chains, args.num_splits = shape["section_size"], shape["chains"], shape["num_splits"]
+        log(f"[bench] autotuned shape {tuned}")
     t = atg.Table(n, psa_plus=True, chains=args.chains, section_size=args.section_size,
                   num_splits=args.num_splits, split=args.split, learned_segments=args.learned_segments)
     t.set_weights(w, stream)
@@ -253,7 +279,7 @@ def main():
         "config": {"workload": "plain PSA+ alias-table build, N=1e8 FP64 Pareto(1.1) weights (seed 42), "
                                "then 1e9 sectioned samples (2^12-row sections) + rejection variant",
                    "n": n, "section_size": st["section_len"], "chains": args.chains, "p": p, "layout": "aos",
-                   "psa_plus": True},
+                   "psa_plus": True, "split": args.split, "autotuned": tuned},
         "roofline": {"bound": "hbm", "kernel": {"partition": "partition_local+partition_place", "sum": "sum_sections",
                                                 "pack": "pack_staged"}[dom],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

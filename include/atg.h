@@ -208,6 +208,18 @@ atg_status atg_batch_sample(atg_batch* b, uint64_t per_table, uint64_t seed, int
 /* Per-table build latency in ns from device timestamps (last build). */
 atg_status atg_batch_latencies(atg_batch* b, double* ns_per_table);
 
+/* ---- autotuner ----------------------------------------------------------- */
+
+/* Chooses section_size, chains and num_splits for these weights on this
+ * device: builds them with each candidate of a small grid (median of `reps`
+ * timed builds after a warm-up) and writes the fastest options to *best (the
+ * other fields come from *base, or the defaults).  *best_ms: the median build
+ * time, negative when the choice came from the ATG_TUNE_CACHE file.
+ * No counterpart in the reference (its shape is compile-time,
+ * PsaPlusNormal.cu:3, AliasTableSplitGpu.cu:58-71,356). */
+atg_status atg_autotune(const void* weights, atg_mem mem, int64_t n, atg_dtype dtype, atg_layout layout,
+                        const atg_build_opts* base, int32_t reps, atg_build_opts* best, double* best_ms);
+
 /* ---- misc --------------------------------------------------------------- */
 const char* atg_last_error(void);
 const char* atg_version(void);
