@@ -159,3 +159,20 @@ def test_all_zero_weights_rejected(gpu):
     t.build()
     with pytest.raises(atg.AtgError):
         t.sync()
+
+
+def test_autotune_returns_valid_shape(gpu, oracle, tmp_path, monkeypatch):
+    """The tuner picks a shape from its grid; that shape builds the oracle's
+    table; a second call with the cache file returns the cached choice."""
+    monkeypatch.setenv("ATG_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    n = 400_000
+    w = atg.generate_weights("pareto", n, seed=3, param=1.1)
+    shape, ms = atg.autotune(w, reps=1)
+    assert ms > 0
+    assert shape["section_size"] in (1024, 1536, 2048) and shape["chains"] in (12, 16, 24, 32)
+    assert 2 <= shape["num_splits"] <= n // 10
+    t = gpu_build(w, chains=shape["chains"], section_size=shape["section_size"], num_splits=shape["num_splits"])
+    ref = oracle.build(w, chains=shape["chains"], section_size=shape["section_size"], num_splits=shape["num_splits"])
+    assert_same_build(t, ref, n)
+    shape2, ms2 = atg.autotune(w, reps=1)
+    assert ms2 < 0 and shape2 == shape
