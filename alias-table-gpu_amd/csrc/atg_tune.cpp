@@ -84,7 +84,7 @@ extern "C" atg_status atg_autotune(const void* weights, atg_mem mem, int64_t n, 
             *best = o;
             best->section_size = std::min<int64_t>(S, std::max<int64_t>(1, n));
             best->chains = C;
-            best->num_splits = std::max<int64_t>(2, (int64_t)std::llround((double)n / ratio));
+            best->num_splits = std::max<int64_t>(2, (int64_t)((double)n / ratio));
             if (best_ms) *best_ms = -ms;  // negative: from the cache, not measured now
             return ATG_OK;
         }
