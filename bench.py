@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=10_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sampling", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first copy a device buffer of this many MiB 3 times (scale for HBM byte counters)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --section-size/--chains/--num-splits instead of the autotuner's choice")
     args = ap.parse_args()
@@ -218,6 +220,16 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calibrate_copy_mib > 0:
+        # A copy of known size: its FETCH_SIZE / WRITE_SIZE counters give the
+        # scale that turns the build kernels' counters into bytes.
+        src = torch.empty(args.calibrate_copy_mib << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"[bench] calibration copies of {args.calibrate_copy_mib} MiB done")
     n = args.n
     log(f"[bench] generating N={n} Pareto(1.1) weights")
     w = atg.generate_weights("pareto", n, seed=42, param=1.1)
