@@ -24,6 +24,11 @@ sys.path.insert(0, os.path.join(ROOT, "alias-table-gpu_amd"))
 
 METRIC = "build Gitems/s + Gsamples/s at N=1e8 (%HBM peak); learned/batched vs plain"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per build phase measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (calibrated on a 1 GiB device copy) for the headline workload at this shape;
+# profiles/build_pareto1e8_s1536_c16_hbm_traffic.md has the commands and counts.
+MEASURED_TRAFFIC = {(100_000_000, 1536, 16, 2631578): {"sum": 804_000_000, "partition": 4_250_000_000,
+                                                       "split": 4_872_000_000, "pack": 1_853_000_000}}
 
 
 def log(*a):
@@ -295,7 +300,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": {"partition": "partition_local+partition_place", "sum": "sum_sections",
                                                 "pack": "pack_staged"}[dom],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((n, args.section_size, args.chains, args.num_splits),
+                                                     {}).get(dom),
                      "algorithmic_bytes": pb[dom], "mean_ms": round(phase_ms[dom], 4)},
         "build": {"phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
                   "R": R, "R_over_N": round(R / n, 4), "P_w": 2, "B_build": b_build,
