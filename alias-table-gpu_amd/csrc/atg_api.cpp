@@ -174,7 +174,8 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     if (o.learned_segments <= 0) o.learned_segments = 4096;
     if (o.section_size <= 0) o.section_size = std::min<int64_t>(1000, n / 10);
     if (o.section_size < 1) o.section_size = 1;
-    if (o.num_splits <= 0) o.num_splits = 32LL * device_cus() * 10;
+    if (o.num_splits < 0) o.num_splits = std::max<int64_t>(2, n / -o.num_splits);  // one split per |num_splits| items
+    if (o.num_splits == 0) o.num_splits = 32LL * device_cus() * 10;
 
     atg_table* t = new atg_table_s();
     t->n = n;

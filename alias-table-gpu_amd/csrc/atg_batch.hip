@@ -308,7 +308,8 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
             return cleanup(fail(ATG_EINVAL, "atg_batch_create: table %lld section length %lld exceeds %lld",
                                 (long long)t, (long long)z.Ls, (long long)kMaxSectionLen));
         b->Ls_max = std::max(b->Ls_max, z.Ls);
-        const int64_t nsplits = o.num_splits > 0 ? o.num_splits : p_auto;
+        const int64_t nsplits = o.num_splits > 0 ? o.num_splits
+                                : o.num_splits < 0 ? std::max<int64_t>(2, n / -o.num_splits) : p_auto;
         z.p_req = std::max<int64_t>(2, std::min<int64_t>(nsplits, n / 10));
         z.half = (z.G + 63) / 64 + 1;
         z.tree_total = block_tree_total(z.G);

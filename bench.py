@@ -149,7 +149,8 @@ def bench_batch(args, stream, name, ntables, lo, hi, dist, dtype, split, per_tab
     import atg
     offsets, w = atg.generate_batch(dist, ntables, lo, hi, seed=42, param=param, dtype=dtype)
     total = int(offsets[-1])
-    b = atg.Batch(offsets, dtype=dtype, split=split)
+    b = atg.Batch(offsets, dtype=dtype, split=split, section_size=args.batch_section, chains=args.batch_chains,
+                  num_splits=-args.batch_items_per_split)
     b.set_weights(w, stream)
     del w
     out = torch.empty(ntables * per_table, dtype=torch.int32, device="cuda")
@@ -175,6 +176,8 @@ def bench_batch(args, stream, name, ntables, lo, hi, dist, dtype, split, per_tab
     row_bytes = 16
     w_bytes = 4 if dtype == "f32" else 8
     res = {"workload": name, "tables": ntables, "items": total, "dtype": dtype, "split": split,
+           "section_size": args.batch_section, "chains": args.batch_chains,
+           "items_per_split": args.batch_items_per_split,
            "build_ms": round(ms, 4), "gitems_s": round(total / (ms * 1e-3) / 1e9, 3),
            "floor_hbm_frac": round(total * (2 * w_bytes + row_bytes) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "samples_per_table": per_table, "sample_ms": round(statistics.mean(sample_ms), 4),
@@ -204,6 +207,9 @@ def main():
     ap.add_argument("--learned-p", type=int, default=2621440, help="p of the learned-vs-plain comparison")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-section", type=int, default=1024, help="section size of every batched table")
+    ap.add_argument("--batch-chains", type=int, default=16)
+    ap.add_argument("--batch-items-per-split", type=int, default=38)
     ap.add_argument("--batch-learned", action="store_true",
                     help="also the 16,384-table FP32 Zipf batch with the learned index (2.4e9 items)")
     ap.add_argument("--cpu-n", type=int, default=10_000_000)

@@ -55,7 +55,8 @@ typedef struct {
     int32_t psa_plus;        /* 1: PSA+ section pre-pack (Variant::psaPlus) */
     int32_t chains;          /* PSA+ greedy chains per section (NUM_LOCAL_SPLITS = 16), 1..64 */
     int64_t section_size;    /* section parameter; 0 = reference: min(1000, N/10) */
-    int64_t num_splits;      /* p (numThreads); 0 = reference AUTO for the chunked pack: 320 * #CU */
+    int64_t num_splits;      /* p (numThreads); 0 = reference AUTO for the chunked pack: 320 * #CU;
+                                negative: one split per -num_splits items (per table in a batch) */
     int32_t split_method;    /* atg_split_method */
     int32_t learned_segments;/* knots of the learned split index (default 4096) */
     int32_t record_phases;   /* 1: hipEvents around sum/partition/prefixsum/split/pack */
