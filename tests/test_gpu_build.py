@@ -176,3 +176,12 @@ def test_autotune_returns_valid_shape(gpu, oracle, tmp_path, monkeypatch):
     assert_same_build(t, ref, n)
     shape2, ms2 = atg.autotune(w, reps=1)
     assert ms2 < 0 and shape2 == shape
+
+
+def test_split_ratio_mode(gpu, oracle):
+    """num_splits < 0 asks for one split per -num_splits items."""
+    n = 300_007
+    w = atg.generate_weights("exponential", n, seed=4)
+    t = gpu_build(w, num_splits=-38)
+    ref = oracle.build(w, num_splits=n // 38)
+    assert_same_build(t, ref, n)
