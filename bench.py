@@ -207,7 +207,7 @@ def main():
     ap.add_argument("--learned-p", type=int, default=2621440, help="p of the learned-vs-plain comparison")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--batch-section", type=int, default=1024, help="section size of every batched table")
+    ap.add_argument("--batch-section", type=int, default=2048, help="section size of every batched table")
     ap.add_argument("--batch-chains", type=int, default=16)
     ap.add_argument("--batch-items-per-split", type=int, default=38)
     ap.add_argument("--batch-learned", action="store_true",
