@@ -4,8 +4,7 @@
 // Every phase is one launch over all tables: a block finds its table and its
 // block within the table from a block->table map built at creation, then runs
 // exactly the device code of the single-table kernels (the same launch shape
-// per table: 16-wave sum blocks, one wave per section, one wave per 64 pack
-// chains).  A table built here is therefore bit-identical to the same weights
+// per table: one wave per section, one wave per 64 pack chains).  A table built here is therefore bit-identical to the same weights
 // built alone with the same options.  Per-table latencies come from the
 // 100 MHz device clock: the first block of the table's sum phase to the last
 // block of its pack.
@@ -77,14 +76,16 @@ __global__ __launch_bounds__(256) void seg_reset_ticks(SegArgs g) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void seg_sum(SegArgs g, SegMap m) {
-    __shared__ double sh[64];
-    __shared__ uint32_t sh_last;
+__global__ __launch_bounds__(256) void seg_sum(SegArgs g, SegMap m) {
     int64_t t, blk;
     seg_locate(m, t, blk);
     if (threadIdx.x == 0) atomicMin((unsigned long long*)&g.ticks[2 * t], (unsigned long long)wall_clock64());
     const BuildArgs& a = g.tables[t];
-    sum_block<T>(a, blk, (a.G + 63) / 64, sh, sh_last);
+    sum_sections_block<T>(a, blk, (a.G + 3) / 4);
+}
+
+__global__ __launch_bounds__(256) void seg_sum_finish(SegArgs g) {
+    sum_finish_block(g.tables[blockIdx.x]);
 }
 
 template <typename T>
@@ -242,7 +243,8 @@ hipError_t launch_batch(const atg_batch* b, hipStream_t s) {
     auto grid = [](const SegMap& m) { return dim3((unsigned)m.blocks); };
     hipLaunchKernelGGL(seg_reset_ticks, dim3((unsigned)std::min<int64_t>((b->ntables + 255) / 256, 4096)), dim3(256),
                        0, s, g);
-    hipLaunchKernelGGL(seg_sum<T>, grid(M[MAP_SUM]), dim3(1024), 0, s, g, M[MAP_SUM]);
+    hipLaunchKernelGGL(seg_sum<T>, grid(M[MAP_SUM]), dim3(256), 0, s, g, M[MAP_SUM]);
+    hipLaunchKernelGGL(seg_sum_finish, dim3((unsigned)b->ntables), dim3(256), 0, s, g);
     hipLaunchKernelGGL(seg_local<T>, grid(M[MAP_LOCAL]), dim3(64), psa_lds_bytes(b->Ls_max), s, g, M[MAP_LOCAL]);
     hipLaunchKernelGGL(seg_place, grid(M[MAP_PLACE]), dim3(256), 0, s, g, M[MAP_PLACE]);
     if (b->opts.split_method == ATG_SPLIT_LEARNED)
@@ -382,7 +384,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         const int64_t n = offsets[t + 1] - offsets[t];
         const int64_t K = std::min<int64_t>(o.learned_segments, z.p_req);
         switch (ph) {
-            case MAP_SUM: return (z.G + 63) / 64;
+            case MAP_SUM: return (z.G + 3) / 4;
             case MAP_LOCAL: return z.G;
             case MAP_PLACE: return (z.G + 3) / 4;
             case MAP_DEG: return degenerate_blocks(n);

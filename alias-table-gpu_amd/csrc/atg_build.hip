@@ -2,9 +2,8 @@
 // AliasTableSplitGpu::build, AliasTableSplitGpu.cu:388-468).
 //
 // Phases (one stream, no host round trips):
-//   1. sum_sections : per-section FP64 sums (64 sections per block), weight
-//                     validation, the first level of the canonical 64-ary tree;
-//                     the last block to finish completes the tree -> W, W/N.
+//   1. sum_sections : per-section FP64 sums (one wave per section), weight
+//                     validation; sum_finish: the canonical 64-ary tree -> W, W/N.
 //   2. partition_local : one wave per section: all loads issued at once,
 //                     stable light/heavy partition into LDS, PSA+ greedy
 //                     chains writing finished rows, leftover order, the
@@ -23,10 +22,12 @@
 namespace atg {
 
 template <typename T>
-__global__ __launch_bounds__(1024) void sum_sections(BuildArgs a) {
-    __shared__ double sh[64];
-    __shared__ uint32_t sh_last;
-    sum_block<T>(a, blockIdx.x, gridDim.x, sh, sh_last);
+__global__ __launch_bounds__(256) void sum_sections(BuildArgs a) {
+    sum_sections_block<T>(a, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(1024) void sum_finish(BuildArgs a) {
+    sum_finish_block(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -84,9 +85,10 @@ __global__ __launch_bounds__(64) void pack_staged(BuildArgs a) {
 static inline unsigned blocks_for(int64_t items, int per) { return (unsigned)((items + per - 1) / per); }
 
 hipError_t launch_sum(const BuildArgs& a, hipStream_t s) {
-    const unsigned g = blocks_for(a.G, 64);
-    if (a.dtype_f32) hipLaunchKernelGGL(sum_sections<float>, dim3(g), dim3(1024), 0, s, a);
-    else hipLaunchKernelGGL(sum_sections<double>, dim3(g), dim3(1024), 0, s, a);
+    const unsigned g = blocks_for(a.G, 4);
+    if (a.dtype_f32) hipLaunchKernelGGL(sum_sections<float>, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(sum_sections<double>, dim3(g), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(sum_finish, dim3(1), dim3(1024), 0, s, a);
     return hipGetLastError();
 }
 

@@ -484,61 +484,46 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
 
 
 // ---------------------------------------------------------------------------
-// W: one block of 16 waves sums 64 sections.
-// Block b owns sections [64 b, 64 b + 64) (16 waves x 4 sections) and writes
-// the tree of their sums (level 1 of the 64-ary tree).  The last block to
-// finish reduces the level-1 values and publishes W.
+// W, pass 1: section sums, one wave per section (block blk of nblk holds
+// sections 4 blk .. 4 blk + 3), and the reset of the dyadic block sums and
+// their arrival counters for the partition.
 template <typename T>
-__device__ __forceinline__ void sum_block(const BuildArgs& a, int64_t blk, int64_t nblk, double* sh,
-                                          uint32_t& sh_last) {
+__device__ __forceinline__ void sum_sections_block(const BuildArgs& a, int64_t blk, int64_t nblk) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const T* w = (const T*)a.weights;
-    for (int q = 0; q < 4; ++q) {
-        const int idx = wave * 4 + q;
-        const int64_t s = blk * 64 + idx;
-        double sum = 0.0;
-        if (s < a.G) sum = section_sum<T>(w, min(a.n, s * a.Ls), min(a.n, (s + 1) * a.Ls), lane);
-        if (lane == 0) sh[idx] = sum;
+    const int64_t s = blk * (blockDim.x >> 6) + wave;
+    if (s < a.G) {
+        const double sum = section_sum<T>(w, min(a.n, s * a.Ls), min(a.n, (s + 1) * a.Ls), lane);
+        if (lane == 0) a.section_sums[s] = sum;
     }
-    // Reset the dyadic block sums (3 values per block) and their arrival
-    // counters for the partition pass.
     const int64_t nvals = 3 * a.tree_total, ncnt = (a.tree_total + 1) / 2;
     for (int64_t i = blk * blockDim.x + threadIdx.x; i < nvals + ncnt; i += nblk * blockDim.x)
         a.slots[i] = (i < nvals) ? ATG_NOT_READY : 0ull;
-    __syncthreads();
-    if (wave == 0) {
-        const double r = wave_tree_sum(sh[lane]);
-        if (lane == 0) {
-            a.tree_scratch[blk] = r;
-            // Release: the level-1 value is visible before the counter moves.
-            const uint32_t done = __hip_atomic_fetch_add(&a.st->sum_done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            sh_last = (done == nblk - 1) ? 1u : 0u;
-        }
-    }
-    __syncthreads();
-    if (!sh_last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // Last block: the remaining levels of the tree over tree_scratch.
+}
+
+// W, pass 2 (one block): the 64-ary tree over the section sums, 64 values per
+// wave and level, zero-padded; W, W/N, the error bits and the state resets.
+__device__ __forceinline__ void sum_finish_block(const BuildArgs& a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     double* bufs[2] = {a.tree_scratch, a.tree_scratch + a.tree_scratch_half};
+    const double* src = a.section_sums;
     int cur = 0;
-    int64_t count = nblk;
-    const int nwaves = blockDim.x >> 6;
-    while (count > 1) {
+    int64_t count = a.G;
+    while (true) {
         const int64_t groups = (count + 63) / 64;
-        const double* src = bufs[cur];
-        double* dst = bufs[cur ^ 1];
-        for (int64_t g = wave; g < groups; g += nwaves) {
+        for (int64_t g = wave; g < groups; g += nw) {
             const int64_t i = g * 64 + lane;
             const double r = wave_tree_sum((i < count) ? src[i] : 0.0);
-            if (lane == 0) dst[g] = r;
+            if (lane == 0) bufs[cur][g] = r;
         }
-        __threadfence_block();
         __syncthreads();
+        src = bufs[cur];
         cur ^= 1;
         count = groups;
+        if (count == 1) break;
     }
     if (threadIdx.x == 0) {
-        const double W = bufs[cur][0];
+        const double W = src[0];
         DevState* st = a.st;
         uint32_t err = 0;
         if (W != W) err = ERR_BAD_WEIGHT;
@@ -553,7 +538,6 @@ __device__ __forceinline__ void sum_block(const BuildArgs& a, int64_t blk, int64
         st->cert_done = 0;
         st->split_fallbacks = 0;
         st->learned_fallbacks = 0;
-        st->sum_done = 0;
     }
 }
 

@@ -32,7 +32,7 @@ struct DevState {
     uint64_t split_fallbacks;
     uint64_t learned_fallbacks;
     uint64_t rej_size;        // |A| of the rejection structure
-    uint32_t sum_done;        // blocks of the sum pass that finished (last one finalises W)
+    uint32_t pad3;
     uint32_t pad1;
 };
 
