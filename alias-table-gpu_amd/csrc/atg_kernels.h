@@ -25,12 +25,12 @@ __device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t
     double acc = 0.0;
     bool bad = false;
     int64_t k = start + lane;
-    for (; k + 7 * 64 < end; k += 8 * 64) {
-        double x[8];
+    for (; k + 15 * 64 < end; k += 16 * 64) {
+        double x[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) x[u] = (double)w[k + u * 64];
+        for (int u = 0; u < 16; ++u) x[u] = (double)w[k + u * 64];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < 16; ++u) {
             bad |= !(x[u] >= 0.0) || (x[u] == __builtin_inf());
             acc = acc + x[u];
         }
@@ -256,60 +256,52 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
             jLo = t * stepsH; jHi = (t == C - 1) ? nH : (t + 1) * stepsH;
         }
         if (t < C && iHi - iLo > 0 && jHi - jLo > 0) {
-            // Registers hold the current light (lw, li), the next light
-            // (lw1, li1), the next two heavies (hw1/hi1, hw2/hi2); the load a
-            // step issues is first needed a step later, so LDS latency stays
-            // off the chain's critical path.  Prefetch indices are clamped to
-            // the chain's range; a clamped value is never consumed (the loop
-            // leaves before).
-            // Lights and heavies share one LDS array (heavy j at len-1-j), so
-            // the one load a step issues has a selected address; its value
-            // (nv, ni) is resolved into lw1 or hw2 only in the next step,
-            // after that step's decision, which needs lw and hw1 alone.
+            // Registers hold the chain's current heavy (residual wc, item
+            // hj), the next heavy (hw1, hi1) and the current light (lw, li).
+            // Every step issues the loads of the light after the current one
+            // (nlw, nli) and of the heavy after the next one (nhw, nhi); the
+            // next step consumes one of the two, so LDS latency stays off the
+            // chain's critical path.  Indices are clamped to the chain's
+            // ranges; a clamped value is never consumed (the loop leaves
+            // first).  Lights and heavies share one LDS array (heavy j at
+            // len-1-j).
             int i = iLo, j = jLo;
             double wc = L.Hw(j);
             int hj = L.Hi(j);
             double lw = L.Lw(i);
             int li = L.Li(i);
-            int q = min(i + 1, iHi - 1);
-            double lw1 = L.Lw(q);
-            int li1 = L.Li(q);
-            q = min(j + 1, jHi - 1);
+            int q = min(j + 1, jHi - 1);
             double hw1 = L.Hw(q);
             int hi1 = L.Hi(q);
+            q = min(i + 1, iHi - 1);
+            double nlw = L.Lw(q);
+            int nli = L.Li(q);
             q = min(j + 2, jHi - 1);
-            double hw2 = L.Hw(q);
-            int hi2 = L.Hi(q);
-            double nv = 0.0;
-            int ni = 0;
-            int pk = 0;  // pending load: 0 none, 1 -> (lw1, li1), 2 -> (hw2, hi2)
+            double nhw = L.Hw(q);
+            int nhi = L.Hi(q);
             const double thr = W_N + ATG_EPSILON;
             while (true) {
                 const bool ex = wc <= thr;
-                if ((ex && j >= jHi - 1) || (!ex && i >= iHi - 1)) break;
+                const int room = ex ? jHi - 1 - j : iHi - 1 - i;
+                if (room <= 0) break;
                 const int item = ex ? hj : li;
                 const double rw = ex ? wc : lw;
                 const int alias = ex ? hi1 : hj;
                 const double next = ex ? hw1 : lw;
                 wc += next - W_N;
                 store_row<AOS>(a, start + item, rw, (int32_t)(start + alias));
-                // The previous step's load completes the prefetch registers.
-                lw1 = pk == 1 ? nv : lw1;
-                li1 = pk == 1 ? ni : li1;
-                hw2 = pk == 2 ? nv : hw2;
-                hi2 = pk == 2 ? ni : hi2;
-                // Advance one side and request the operand two steps ahead.
                 j += ex ? 1 : 0;
                 i += ex ? 0 : 1;
                 hj = ex ? hi1 : hj;
-                hw1 = ex ? hw2 : hw1;
-                hi1 = ex ? hi2 : hi1;
-                lw = ex ? lw : lw1;
-                li = ex ? li : li1;
-                const int pos = ex ? len - 1 - min(j + 2, jHi - 1) : min(i + 1, iHi - 1);
-                nv = L.w[pos];
-                ni = L.idx[pos];
-                pk = ex ? 2 : 1;
+                hw1 = ex ? nhw : hw1;
+                hi1 = ex ? nhi : hi1;
+                lw = ex ? lw : nlw;
+                li = ex ? li : nli;
+                const int qi = min(i + 1, iHi - 1), qj = len - 1 - min(j + 2, jHi - 1);
+                nlw = L.w[qi];
+                nli = L.idx[qi];
+                nhw = L.w[qj];
+                nhi = L.idx[qj];
             }
             // Remainder (PsaPlusNormal.cu:112-123): back to L when its
             // residual is light and a consumed light slot can hold it,
@@ -511,10 +503,22 @@ __device__ __forceinline__ void sum_finish_block(const BuildArgs& a) {
     int64_t count = a.G;
     while (true) {
         const int64_t groups = (count + 63) / 64;
-        for (int64_t g = wave; g < groups; g += nw) {
-            const int64_t i = g * 64 + lane;
-            const double r = wave_tree_sum((i < count) ? src[i] : 0.0);
-            if (lane == 0) bufs[cur][g] = r;
+        // Wave w reduces groups w, w + nw, ...; eight groups' loads in flight.
+        for (int64_t g0 = wave; g0 < groups; g0 += 8 * nw) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int64_t i = (g0 + u * nw) * 64 + lane;
+                v[u] = (i < count) ? src[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int64_t g = g0 + u * nw;
+                if (g < groups) {
+                    const double r = wave_tree_sum(v[u]);
+                    if (lane == 0) bufs[cur][g] = r;
+                }
+            }
         }
         __syncthreads();
         src = bufs[cur];
