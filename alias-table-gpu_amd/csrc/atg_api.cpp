@@ -116,6 +116,7 @@ struct atg_table_s {
         a.chains = opts.chains;
         a.p_req = p_req;
         a.splits = d_splits;
+        a.pack_cap = pack_cap_items(n, p_req);
         a.learned_segments = opts.learned_segments;
         a.knot_j = d_knots;
         return a;

@@ -199,6 +199,7 @@ struct atg_batch_s {
     std::vector<int64_t> offsets;
     std::vector<BuildArgs> args;        // host copy (pointers into the pools below)
     int64_t Ls_max = 1;
+    int32_t pack_cap_max = 16;          // largest pack LDS pool of the tables (items)
     double wall_khz = 100000.0;
     SegMap maps[MAP_COUNT] = {};
     // device memory
@@ -256,7 +257,7 @@ hipError_t launch_batch(const atg_batch* b, hipStream_t s) {
     } else {
         hipLaunchKernelGGL(seg_split, grid(M[MAP_SPLIT]), dim3(256), 0, s, g, M[MAP_SPLIT]);
     }
-    hipLaunchKernelGGL(seg_pack, grid(M[MAP_PACK]), dim3(64), (size_t)PACK_CAP * 24, s, g, M[MAP_PACK]);
+    hipLaunchKernelGGL(seg_pack, grid(M[MAP_PACK]), dim3(64), pack_lds_bytes(b->pack_cap_max), s, g, M[MAP_PACK]);
     return hipGetLastError();
 }
 
@@ -374,6 +375,8 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         a.psa_plus = o.psa_plus;
         a.chains = o.chains;
         a.p_req = z.p_req;
+        a.pack_cap = pack_cap_items(n, z.p_req);
+        b->pack_cap_max = std::max(b->pack_cap_max, a.pack_cap);
         a.splits = b->d_splits + osp[t];
         a.learned_segments = o.learned_segments;
     }

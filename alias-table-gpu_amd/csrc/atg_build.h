@@ -34,6 +34,7 @@ struct BuildArgs {
     int32_t chains;
     int64_t p_req;            // p after the N/10 clamp (grid bound for split/pack)
     Split* splits;            // capacity p_req + 1
+    int32_t pack_cap;         // list items one pack wave stages in LDS (pack_cap_items)
     // learned split index
     int32_t learned_segments;
     int32_t* knot_j;          // learned_segments + 1 exact split j at the knots
@@ -45,6 +46,21 @@ inline void section_layout(int64_t n, int64_t param, int64_t* G, int64_t* Ls) {
     *G = n / param + 1;
     *Ls = n / *G + 1;
 }
+
+// Items one pack wave stages in LDS.  Splits are evenly spaced in list items
+// (n' = ceil(nToSplit k / p)), so the 64 chains of a wave hold at most
+// 64 ceil(nToSplit / p) + 1 items; nToSplit <= n + 2, and p falls below p_req
+// only through the N/10 and R/10 clamps, which keep chains near 10 items.  A
+// wave whose items exceed the capacity reads the lists from HBM instead.
+constexpr int32_t kPackMaxItems = 4096;
+inline int32_t pack_cap_items(int64_t n, int64_t p_req) {
+    if (p_req < 1) p_req = 1;
+    int64_t per = (n + 2 + p_req - 1) / p_req;
+    if (per < 11) per = 11;
+    int64_t cap = (64 * (per + 1) + 2 + 15) / 16 * 16;
+    return (int32_t)(cap < kPackMaxItems ? cap : kPackMaxItems);
+}
+inline size_t pack_lds_bytes(int32_t cap) { return (size_t)cap * 12; }
 
 size_t psa_lds_bytes(int64_t Ls_max);
 int64_t block_tree_total(int64_t G);

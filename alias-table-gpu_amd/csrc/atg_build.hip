@@ -133,7 +133,7 @@ hipError_t launch_split(const BuildArgs& a, hipStream_t s) {
 
 hipError_t launch_pack(const BuildArgs& a, hipStream_t s) {
     const int64_t kmax = a.p_req;
-    const size_t lds = (size_t)PACK_CAP * 24;
+    const size_t lds = pack_lds_bytes(a.pack_cap);
     if (a.rows) hipLaunchKernelGGL(pack_staged<true>, dim3(blocks_for(kmax, 64)), dim3(64), lds, s, a);
     else hipLaunchKernelGGL(pack_staged<false>, dim3(blocks_for(kmax, 64)), dim3(64), lds, s, a);
     return hipGetLastError();

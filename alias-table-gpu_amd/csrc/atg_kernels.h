@@ -10,10 +10,14 @@
 
 namespace atg {
 
+// Orders one wave's LDS writes before its later LDS reads by other lanes.  A
+// wave's LDS operations execute in issue order, so wavefront-scope fences
+// (compiler barriers, no wait instructions) suffice; a workgroup-scope fence
+// would also wait for every outstanding global store of the wave.
 __device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ---------------------------------------------------------------------------
@@ -179,18 +183,34 @@ __device__ __forceinline__ Prefix block_prefix_done(const BuildArgs& a, int64_t 
     return P;
 }
 
-// Position of leftover rank r of one class: chain t (lane t) holds its
-// leftovers at [lo_t, lo_t + count_t) and its ranks start at off_t (exclusive
-// scan of the counts, non-decreasing in t; lanes without a chain hold the
-// total).  The chain of r is the largest t with off_t <= r.  All lanes take part.
-__device__ __forceinline__ int chain_position(int r, int off, int lo) {
-    int t = 0;
+// Positions of leftover ranks r0 + 64 u (u < U) of one class: chain t (lane
+// t) holds its leftovers at [lo_t, lo_t + count_t) and its ranks start at
+// off_t (exclusive scan of the counts, non-decreasing in t; lanes without a
+// chain hold the total).  The chain of rank r is the largest t with
+// off_t <= r, found by a binary search over the lanes; the U searches run
+// interleaved, so each step issues U lane reads before it waits.  All lanes
+// take part.
+template <int U>
+__device__ __forceinline__ void chain_positions(int r0, int off, int lo, int* pos) {
+    int t[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) t[u] = 0;
 #pragma unroll
     for (int step = 32; step >= 1; step >>= 1) {
-        const int o = __shfl(off, t + step, 64);
-        if (o <= r) t += step;
+        int o[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) o[u] = __shfl(off, t[u] + step, 64);
+#pragma unroll
+        for (int u = 0; u < U; ++u) t[u] += (o[u] <= r0 + 64 * u) ? step : 0;
     }
-    return __shfl(lo, t, 64) + (r - __shfl(off, t, 64));
+    int b[U], c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        b[u] = __shfl(lo, t[u], 64);
+        c[u] = __shfl(off, t[u], 64);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) pos[u] = b[u] + (r0 + 64 * u - c[u]);
 }
 
 // Partition pass 1, section s by one wave: stable light/heavy partition into
@@ -221,6 +241,10 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
         const int k = (u << 6) + lane;
         x[u] = (k < len) ? (double)w[start + k] : 0.0;
     }
+    // vmcnt(0) on every path: without it the compiler keeps the loads
+    // pending on the skipped blocks below and waits later, when the
+    // registers are reused, for every row store the chains issued by then.
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     int nL = 0, nH = 0;
 #pragma unroll
     for (int u = 0; u < PSA_MAX_LEN / 64; ++u) {
@@ -339,14 +363,20 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
         const int off = heavy ? offH : offL, lo = heavy ? jLo : iLo;
         for (int g0 = 0; g0 < m; g0 += 64 * PLACE_GROUP) {
             double v[PLACE_GROUP];
+            int pos[PLACE_GROUP];
+            if (identity) {
+#pragma unroll
+                for (int u = 0; u < PLACE_GROUP; ++u) pos[u] = g0 + 64 * u + lane;
+            } else {
+                chain_positions<PLACE_GROUP>(g0 + lane, off, lo, pos);
+            }
 #pragma unroll
             for (int u = 0; u < PLACE_GROUP; ++u) {
                 const int r = g0 + 64 * u + lane;
-                const int pos = identity ? r : chain_position(r, off, lo);
                 v[u] = 0.0;
                 if (r < m) {
-                    v[u] = heavy ? L.Hw(pos) : L.Lw(pos);
-                    const int it = heavy ? L.Hi(pos) : L.Li(pos);
+                    v[u] = heavy ? L.Hw(pos[u]) : L.Lw(pos[u]);
+                    const int it = heavy ? L.Hi(pos[u]) : L.Li(pos[u]);
                     a.stage_w[start + rank0 + r] = v[u];
                     a.stage_i[start + rank0 + r] = (int32_t)(start + it);
                 }
@@ -548,7 +578,6 @@ __device__ __forceinline__ void sum_finish_block(const BuildArgs& a) {
 
 // ---------------------------------------------------------------------------
 // Pack helpers.
-#define PACK_CAP 2048
 
 struct GlobalLists {
     const int32_t *Li, *Hi;
@@ -568,29 +597,54 @@ struct StagedLists {
     __device__ double hw(int j) const { return Hw[j - jA]; }
 };
 
-// One pack chain (PackMethodBasic.cuh:59-126); hj caches the current heavy item.
+// One pack chain (PackMethodBasic.cuh:59-126).  Registers hold the current
+// bucket weight w, the current heavy item hj, the next heavy (hw1, hi1) and
+// the current light (lw, li); every step issues the loads of the light after
+// the current one and of the heavy after the next one, which the following
+// step consumes at most one of, so list reads stay off the chain's critical
+// path.  Read indices are clamped to the chain's own ranges (lights up to
+// max(iUpper - 1, prev.i), heavies up to jUpper); a clamped value is never
+// consumed.
 template <bool AOS, typename Lists>
 __device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, Split prev, Split cur, double W_N) {
     const int iUpper = cur.i, jUpper = cur.j;
     int i = prev.i, j = prev.j;
+    const int iLast = max(iUpper - 1, i);
     double w = prev.spill;
     if (prev.spill == 0) w = l.hw(j);
     int hj = l.hi(j);
+    double lw = l.lw(min(i, iLast));
+    int li = l.li(min(i, iLast));
+    double hw1 = l.hw(min(j + 1, jUpper));
+    int hi1 = l.hi(min(j + 1, jUpper));
+    double nlw = l.lw(min(i + 1, iLast));
+    int nli = l.li(min(i + 1, iLast));
+    double nhw = l.hw(min(j + 2, jUpper));
+    int nhi = l.hi(min(j + 2, jUpper));
+    const double thr = W_N + ATG_EPSILON;
     while (true) {
-        bool ex = w <= W_N + ATG_EPSILON;
-        if (ex && j >= jUpper) ex = false;
-        else if (!ex && i >= iUpper) ex = true;
-        if ((ex && j >= jUpper) || (!ex && i >= iUpper)) break;
-        int32_t item, alias;
-        double rw, next;
-        if (ex) {
-            const int hn = l.hi(j + 1);
-            item = hj; rw = w; alias = hn; next = l.hw(j + 1); ++j; hj = hn;
-        } else {
-            item = l.li(i); rw = l.lw(i); alias = hj; next = rw; ++i;
-        }
+        const bool jdone = j >= jUpper, idone = i >= iUpper;
+        // A side that ran out hands the step to the other one.
+        const bool ex = (w <= thr) ? !jdone : idone;
+        if (ex ? jdone : idone) break;
+        const int item = ex ? hj : li;
+        const double rw = ex ? w : lw;
+        const int alias = ex ? hi1 : hj;
+        const double next = ex ? hw1 : lw;
         w += next - W_N;
         if ((uint32_t)item < (uint32_t)a.n) store_row<AOS>(a, item, rw, alias);
+        j += ex ? 1 : 0;
+        i += ex ? 0 : 1;
+        hj = ex ? hi1 : hj;
+        hw1 = ex ? nhw : hw1;
+        hi1 = ex ? nhi : hi1;
+        lw = ex ? lw : nlw;
+        li = ex ? li : nli;
+        const int qi = min(i + 1, iLast), qj = min(j + 2, jUpper);
+        nlw = l.lw(qi);
+        nli = l.li(qi);
+        nhw = l.hw(qj);
+        nhi = l.hi(qj);
     }
 }
 
@@ -816,40 +870,29 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
     }
     const int nLu = hi.i - lo.i, nHu = hi.j - lo.j + 1;
     // Stage only when every chain of the wave stays inside the copied ranges
-    // (always, unless a bisection fell back).
-    const bool staged = __ballot(!inside) == 0ull && nLu >= 0 && nLu <= PACK_CAP && nHu >= 1 && nHu <= PACK_CAP;
-    double* sLw = lds_raw;
-    double* sHw = sLw + PACK_CAP;
-    int32_t* sLi = reinterpret_cast<int32_t*>(sHw + PACK_CAP);
-    int32_t* sHi = sLi + PACK_CAP;
+    // (always, unless a bisection fell back) and the ranges fit the pool:
+    // weights at sW, items at sI, lights at [0, nLu), heavies after them.
+    const bool staged = __ballot(!inside) == 0ull && nLu >= 0 && nHu >= 1 && nLu + nHu <= a.pack_cap;
+    double* sW = lds_raw;
+    int32_t* sI = reinterpret_cast<int32_t*>(sW + a.pack_cap);
     if (staged) {
-        const int iA = lo.i, jA = lo.j;
-        for (int t0 = 0; t0 < nLu; t0 += 8 * 64) {
-            double wv[8];
-            int32_t iv[8];
+        for (int hv = 0; hv < 2; ++hv) {
+            const int m = hv ? nHu : nLu, base = hv ? nLu : 0;
+            const double* gw = hv ? a.Hw + lo.j : a.Lw + lo.i;
+            const int32_t* gi = hv ? a.Hi + lo.j : a.Li + lo.i;
+            for (int t0 = 0; t0 < m; t0 += 8 * 64) {
+                double wv[8];
+                int32_t iv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int t = t0 + u * 64 + lane;
-                if (t < nLu) { wv[u] = a.Lw[iA + t]; iv[u] = a.Li[iA + t]; }
-            }
+                for (int u = 0; u < 8; ++u) {
+                    const int t = t0 + u * 64 + lane;
+                    if (t < m) { wv[u] = gw[t]; iv[u] = gi[t]; }
+                }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int t = t0 + u * 64 + lane;
-                if (t < nLu) { sLw[t] = wv[u]; sLi[t] = iv[u]; }
-            }
-        }
-        for (int t0 = 0; t0 < nHu; t0 += 8 * 64) {
-            double wv[8];
-            int32_t iv[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int t = t0 + u * 64 + lane;
-                if (t < nHu) { wv[u] = a.Hw[jA + t]; iv[u] = a.Hi[jA + t]; }
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int t = t0 + u * 64 + lane;
-                if (t < nHu) { sHw[t] = wv[u]; sHi[t] = iv[u]; }
+                for (int u = 0; u < 8; ++u) {
+                    const int t = t0 + u * 64 + lane;
+                    if (t < m) { sW[base + t] = wv[u]; sI[base + t] = iv[u]; }
+                }
             }
         }
         wave_lds_sync();
@@ -857,7 +900,7 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
     if (k > kE) return;
     const double W_N = st->W_N;
     if (staged) {
-        const StagedLists l{sLi, sHi, sLw, sHw, lo.i, lo.j};
+        const StagedLists l{sI, sI + nLu, sW, sW + nLu, lo.i, lo.j};
         pack_chain<AOS>(a, l, prev, cur, W_N);
     } else {
         const GlobalLists l{a.Li, a.Hi, a.Lw, a.Hw};
