@@ -88,14 +88,41 @@ __global__ __launch_bounds__(256) void seg_sum_finish(SegArgs g) {
     sum_finish_block(g.tables[blockIdx.x]);
 }
 
+// Sections of all tables, grid-stride over the phase's blocks (one section
+// each); a wave loads its next section while the current one's chains run.
 template <typename T>
 __global__ __launch_bounds__(64) void seg_local(SegArgs g, SegMap m) {
     extern __shared__ double lds_raw[];
+    const int lane = (int)threadIdx.x;
+    double x[PSA_MAX_LEN / 64];
+    auto load = [&](int64_t v) {
+        const int64_t t = m.tab[v];
+        const BuildArgs& a = g.tables[t];
+        if (!a.st->error) section_load<T>(a, v - m.base[t], lane, x);
+    };
+    int64_t v = blockIdx.x;
+    if (v < m.blocks) load(v);
+    for (; v < m.blocks; v += gridDim.x) {
+        const int64_t t = m.tab[v], vn = v + gridDim.x;
+        const BuildArgs& a = g.tables[t];
+        auto next = [&]() {
+            if (vn < m.blocks) load(vn);
+        };
+        if (a.st->error) {
+            next();
+            continue;
+        }
+        psa_section<true>(a, v - m.base[t], x, lds_raw, lane, next);
+    }
+}
+
+// Levels b0 + 1 .. b0 + 9 of every table's dyadic block tree.
+__global__ __launch_bounds__(256) void seg_tree(SegArgs g, SegMap m, int b0) {
     int64_t t, blk;
     seg_locate(m, t, blk);
     const BuildArgs& a = g.tables[t];
     if (a.st->error) return;
-    psa_local<T, true>(a, blk, lds_raw, (int)threadIdx.x);
+    tree_block(a, blk, b0);
 }
 
 __global__ __launch_bounds__(256) void seg_place(SegArgs g, SegMap m) {
@@ -189,7 +216,7 @@ __global__ __launch_bounds__(256) void batch_sample(BatchSampleArgs a) {
 
 using namespace atg;
 
-enum { MAP_SUM, MAP_LOCAL, MAP_PLACE, MAP_DEG, MAP_SPLIT, MAP_KNOTS, MAP_PACK, MAP_COUNT };
+enum { MAP_SUM, MAP_LOCAL, MAP_PLACE, MAP_DEG, MAP_SPLIT, MAP_KNOTS, MAP_PACK, MAP_TREE, MAP_COUNT };
 
 struct atg_batch_s {
     int64_t ntables = 0, total = 0;
@@ -200,6 +227,7 @@ struct atg_batch_s {
     std::vector<BuildArgs> args;        // host copy (pointers into the pools below)
     int64_t Ls_max = 1;
     int32_t pack_cap_max = 16;          // largest pack LDS pool of the tables (items)
+    int64_t G_max = 1;                  // most sections of a table
     double wall_khz = 100000.0;
     SegMap maps[MAP_COUNT] = {};
     // device memory
@@ -246,7 +274,12 @@ hipError_t launch_batch(const atg_batch* b, hipStream_t s) {
                        0, s, g);
     hipLaunchKernelGGL(seg_sum<T>, grid(M[MAP_SUM]), dim3(256), 0, s, g, M[MAP_SUM]);
     hipLaunchKernelGGL(seg_sum_finish, dim3((unsigned)b->ntables), dim3(256), 0, s, g);
-    hipLaunchKernelGGL(seg_local<T>, grid(M[MAP_LOCAL]), dim3(64), psa_lds_bytes(b->Ls_max), s, g, M[MAP_LOCAL]);
+    const size_t lds = psa_lds_bytes(b->Ls_max);
+    const int64_t resident = resident_blocks((const void*)seg_local<T>, 64, lds);
+    hipLaunchKernelGGL(seg_local<T>, dim3((unsigned)std::min<int64_t>(M[MAP_LOCAL].blocks, resident)), dim3(64), lds,
+                       s, g, M[MAP_LOCAL]);
+    for (int b0 = 0; (b->G_max >> b0) > 1; b0 += 9)
+        hipLaunchKernelGGL(seg_tree, grid(M[MAP_TREE]), dim3(256), 0, s, g, M[MAP_TREE], b0);
     hipLaunchKernelGGL(seg_place, grid(M[MAP_PLACE]), dim3(256), 0, s, g, M[MAP_PLACE]);
     if (b->opts.split_method == ATG_SPLIT_LEARNED)
         hipLaunchKernelGGL(seg_cert, dim3((unsigned)b->ntables), dim3(256), 0, s, g);
@@ -377,6 +410,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         a.p_req = z.p_req;
         a.pack_cap = pack_cap_items(n, z.p_req);
         b->pack_cap_max = std::max(b->pack_cap_max, a.pack_cap);
+        b->G_max = std::max(b->G_max, z.G);
         a.splits = b->d_splits + osp[t];
         a.learned_segments = o.learned_segments;
     }
@@ -390,6 +424,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
             case MAP_SUM: return (z.G + 3) / 4;
             case MAP_LOCAL: return z.G;
             case MAP_PLACE: return (z.G + 3) / 4;
+            case MAP_TREE: return (z.G + 511) / 512;
             case MAP_DEG: return degenerate_blocks(n);
             case MAP_SPLIT: return std::max<int64_t>(1, (z.p_req - 1 + 255) / 256);
             case MAP_KNOTS: return (K + 1 + 255) / 256;

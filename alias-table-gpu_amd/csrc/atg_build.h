@@ -63,6 +63,7 @@ inline int32_t pack_cap_items(int64_t n, int64_t p_req) {
 inline size_t pack_lds_bytes(int32_t cap) { return (size_t)cap * 12; }
 
 size_t psa_lds_bytes(int64_t Ls_max);
+int64_t resident_blocks(const void* f, int threads, size_t lds);
 int64_t block_tree_total(int64_t G);
 hipError_t set_partition_lds_limit(size_t bytes);
 hipError_t launch_sum(const BuildArgs& a, hipStream_t s);

@@ -36,11 +36,28 @@ __global__ __launch_bounds__(1024) void sum_finish(BuildArgs a) {
 //    section publishes its aggregate as the level-0 block, the child that
 //    completes a pair second publishes the parent (pass 1), and P(s) folds
 //    the blocks of s's binary digits from high to low (pass 2).
+// As many waves as fit the chip at once; wave w runs sections w, w + grid,
+// ... and loads each next section while the current one's chains run.
 template <typename T, bool AOS>
 __global__ __launch_bounds__(64) void partition_local(BuildArgs a) {
     extern __shared__ double lds_raw[];
     if (a.st->error) return;
-    psa_local<T, AOS>(a, (int64_t)blockIdx.x, lds_raw, (int)threadIdx.x);
+    const int lane = (int)threadIdx.x;
+    double x[PSA_MAX_LEN / 64];
+    int64_t s = blockIdx.x;
+    if (s < a.G) section_load<T>(a, s, lane, x);
+    for (; s < a.G; s += gridDim.x) {
+        const int64_t sn = s + gridDim.x;
+        psa_section<AOS>(a, s, x, lds_raw, lane, [&]() {
+            if (sn < a.G) section_load<T>(a, sn, lane, x);
+        });
+    }
+}
+
+// Levels b0 + 1 .. b0 + 9 of the dyadic block tree.
+__global__ __launch_bounds__(256) void block_tree(BuildArgs a, int b0) {
+    if (a.st->error) return;
+    tree_block(a, blockIdx.x, b0);
 }
 
 // The split certificate from the per-section data.
@@ -93,6 +110,18 @@ hipError_t launch_sum(const BuildArgs& a, hipStream_t s) {
 }
 
 // LDS of one partition wave: weights (8 B) and local item indices (2 B).
+// Workgroups of `threads` threads and `lds` bytes of dynamic LDS of kernel f
+// that the device holds at once.
+int64_t resident_blocks(const void* f, int threads, size_t lds) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, threads, lds) != hipSuccess || per < 1) {
+        (void)hipGetLastError();
+        return int64_t(1) << 30;  // unknown: one block per section
+    }
+    return (int64_t)cus * per;
+}
+
 size_t psa_lds_bytes(int64_t Ls_max) { return (((size_t)Ls_max * 10) + 15) & ~(size_t)15; }
 
 int64_t block_tree_total(int64_t G) {
@@ -103,14 +132,12 @@ int64_t block_tree_total(int64_t G) {
 
 hipError_t launch_partition(const BuildArgs& a, hipStream_t s) {
     const size_t lds = psa_lds_bytes(a.Ls_max);
-    const dim3 g((unsigned)a.G), b(64);
-    if (a.dtype_f32) {
-        if (a.rows) hipLaunchKernelGGL((partition_local<float, true>), g, b, lds, s, a);
-        else hipLaunchKernelGGL((partition_local<float, false>), g, b, lds, s, a);
-    } else {
-        if (a.rows) hipLaunchKernelGGL((partition_local<double, true>), g, b, lds, s, a);
-        else hipLaunchKernelGGL((partition_local<double, false>), g, b, lds, s, a);
-    }
+    void (*k)(BuildArgs) = a.dtype_f32 ? (a.rows ? partition_local<float, true> : partition_local<float, false>)
+                                       : (a.rows ? partition_local<double, true> : partition_local<double, false>);
+    const dim3 g((unsigned)std::min<int64_t>(a.G, resident_blocks((const void*)k, 64, lds)));
+    hipLaunchKernelGGL(k, g, dim3(64), lds, s, a);
+    for (int b0 = 0; (a.G >> b0) > 1; b0 += 9)
+        hipLaunchKernelGGL(block_tree, dim3(blocks_for(a.G >> b0, 512)), dim3(256), 0, s, a, b0);
     hipLaunchKernelGGL(partition_place, dim3(blocks_for(a.G, 4)), dim3(256), 0, s, a);
     if (a.want_cert)
         hipLaunchKernelGGL(cert_reduce, dim3(blocks_for(a.G, 256 * 16)), dim3(256), 0, s, a);

@@ -94,20 +94,8 @@ __device__ __forceinline__ int64_t level_offset(int64_t G, int b) {
     return off;
 }
 
-__device__ __forceinline__ void block_load(const BuildArgs& a, int64_t idx, uint64_t& c, uint64_t& l, uint64_t& h,
-                                           bool& timeout) {
-    const uint64_t* V = a.slots;
-    uint32_t spins = 0;
-    while (true) {
-        c = slot_load(V + idx);
-        l = slot_load(V + a.tree_total + idx);
-        h = slot_load(V + 2 * a.tree_total + idx);
-        if (c != ATG_NOT_READY && l != ATG_NOT_READY && h != ATG_NOT_READY) return;
-        if (++spins > (1u << 18)) { timeout = true; c = 0; l = d2u(0.0); h = d2u(0.0); return; }
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
-
+// Level-b block c (all levels in one array per quantity) at slot
+// level_offset(G, b) + c.
 __device__ __forceinline__ void block_store(const BuildArgs& a, int64_t idx, uint64_t c, double l, double h) {
     uint64_t* V = a.slots;
     slot_store(V + idx, c);
@@ -115,54 +103,9 @@ __device__ __forceinline__ void block_store(const BuildArgs& a, int64_t idx, uin
     slot_store(V + 2 * a.tree_total + idx, d2u(h));
 }
 
-// Publish section s's aggregate and complete the parents for which it is
-// the second child to arrive (single lane).
-__device__ void block_publish(const BuildArgs& a, int64_t s, uint64_t c, double l, double h, bool& timeout) {
-    const int64_t G = a.G;
-    block_store(a, s, c, l, h);
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(a.slots + 3 * a.tree_total);
-    int64_t child = s, off_child = 0, off_parent = G;
-    for (int b = 1; ((int64_t)1 << b) <= G; ++b) {
-        const int64_t parent = child >> 1;
-        if (((parent + 1) << b) > G) break;  // incomplete block: never needed
-        const uint32_t old = __hip_atomic_fetch_add(cnt + off_parent + parent, 1u, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) break;  // the sibling completes this block
-        uint64_t lc, ll, lh, rc, rl, rh;
-        block_load(a, off_child + 2 * parent, lc, ll, lh, timeout);
-        block_load(a, off_child + 2 * parent + 1, rc, rl, rh, timeout);
-        c = lc + rc;
-        l = u2d(ll) + u2d(rl);
-        h = u2d(lh) + u2d(rh);
-        block_store(a, off_parent + parent, c, l, h);
-        child = parent;
-        off_child = off_parent;
-        off_parent += G >> b;
-    }
-}
-
-// P(s): lane b loads the block of bit b of s, then the blocks are folded from
-// the highest bit to the lowest.
-__device__ Prefix block_prefix(const BuildArgs& a, int64_t s, int lane, bool& timeout) {
-    uint64_t c = 0, l = d2u(0.0), h = d2u(0.0);
-    const bool set = lane < 62 && ((s >> lane) & 1);
-    if (set) {
-        const int64_t pos = (s >> (lane + 1)) << (lane + 1);
-        block_load(a, level_offset(a.G, lane) + (pos >> lane), c, l, h, timeout);
-    }
-    const uint64_t bits = __ballot(set);
-    Prefix P{0, 0.0, 0.0};
-    for (int b = 63; b >= 0; --b) {
-        if (!((bits >> b) & 1)) continue;
-        P.c += readlane_u64(c, b);
-        P.l = P.l + u2d(readlane_u64(l, b));
-        P.h = P.h + u2d(readlane_u64(h, b));
-    }
-    return P;
-}
-
-// P(s) from block sums completed by an earlier launch: plain loads, no
-// waiting (same fold as block_prefix).
+// P(s): lane b loads the block of bit b of s (all complete, written by
+// earlier launches), then the blocks are folded from the highest bit to the
+// lowest.
 __device__ __forceinline__ Prefix block_prefix_done(const BuildArgs& a, int64_t s, int lane) {
     uint64_t c = 0, l = d2u(0.0), h = d2u(0.0);
     const bool set = lane < 62 && ((s >> lane) & 1);
@@ -181,6 +124,52 @@ __device__ __forceinline__ Prefix block_prefix_done(const BuildArgs& a, int64_t 
         P.h = P.h + u2d(readlane_u64(h, b));
     }
     return P;
+}
+
+// Levels b0 + 1 .. b0 + 9 of the dyadic block tree, T(b, c) = T(b - 1, 2c) +
+// T(b - 1, 2c + 1), over the level-b0 blocks [512 q, 512 q + 512) (one
+// block of 256 threads; only complete blocks, c < G >> b, exist).
+__device__ __forceinline__ void tree_block(const BuildArgs& a, int64_t q, int b0) {
+    __shared__ uint64_t sc[512];
+    __shared__ double sl[512], sh[512];
+    const int64_t G = a.G, T = a.tree_total;
+    uint64_t* V = a.slots;
+    const int64_t cnt0 = G >> b0, first = q * 512;
+    if (first >= cnt0) return;
+    int64_t off = level_offset(G, b0);
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+        const int64_t c = first + i;
+        if (c < cnt0) {
+            sc[i] = V[off + c];
+            sl[i] = u2d(V[T + off + c]);
+            sh[i] = u2d(V[2 * T + off + c]);
+        }
+    }
+    __syncthreads();
+    for (int d = 1; d <= 9; ++d) {
+        const int b = b0 + d;
+        const int64_t cntb = G >> b;
+        if (cntb == 0) break;
+        off += G >> (b - 1);
+        const int nodes = 512 >> d, i = threadIdx.x;
+        const int64_t c = q * nodes + i;
+        const bool valid = i < nodes && c < cntb;
+        uint64_t vc = 0;
+        double vl = 0.0, vh = 0.0;
+        if (valid) {
+            vc = sc[2 * i] + sc[2 * i + 1];
+            vl = sl[2 * i] + sl[2 * i + 1];
+            vh = sh[2 * i] + sh[2 * i + 1];
+        }
+        __syncthreads();
+        if (valid) {
+            sc[i] = vc;
+            sl[i] = vl;
+            sh[i] = vh;
+            block_store(a, off + c, vc, vl, vh);
+        }
+        __syncthreads();
+    }
 }
 
 // Positions of leftover ranks r0 + 64 u (u < U) of one class: chain t (lane
@@ -213,16 +202,31 @@ __device__ __forceinline__ void chain_positions(int r0, int off, int lo, int* po
     for (int u = 0; u < U; ++u) pos[u] = b[u] + (r0 + 64 * u - c[u]);
 }
 
-// Partition pass 1, section s by one wave: stable light/heavy partition into
-// LDS, PSA+ greedy chains writing finished rows, the leftover order, the
-// canonical local-scan aggregates (published as the level-0 block of the
-// dyadic section tree; the second child of a pair to arrive completes the
-// parent) and the leftovers staged in rank order at the section's own range
-// (lights, then heavies).  Nothing here waits for another section's work.
-template <typename T, bool AOS>
-__device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double* lds_raw, int lane) {
-    DevState* st = a.st;
+// The weights of section s, lane-strided (item 64 u + lane in x[u]), as
+// FP64; all loads in flight at once.
+template <typename T>
+__device__ __forceinline__ void section_load(const BuildArgs& a, int64_t s, int lane, double (&x)[PSA_MAX_LEN / 64]) {
     const T* w = (const T*)a.weights;
+    const int64_t start = min(a.n, s * a.Ls);
+    const int len = (int)(min(a.n, (s + 1) * a.Ls) - start);
+#pragma unroll
+    for (int u = 0; u < PSA_MAX_LEN / 64; ++u) {
+        const int k = (u << 6) + lane;
+        x[u] = (k < len) ? (double)w[start + k] : 0.0;
+    }
+}
+
+// Partition pass 1, section s by one wave, its weights in x (section_load):
+// stable light/heavy partition into LDS, then after_partition() (x is free
+// from there on: the caller loads its next section into it), PSA+ greedy
+// chains writing finished rows, the leftover order, the canonical local-scan
+// aggregates (stored as the level-0 block of the dyadic section tree) and the
+// leftovers staged in rank order at the section's own range (lights, then
+// heavies).  Nothing here waits for another section's work.
+template <bool AOS, typename AfterPartition>
+__device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, double (&x)[PSA_MAX_LEN / 64],
+                                            double* lds_raw, int lane, AfterPartition after_partition) {
+    DevState* st = a.st;
     const int64_t start = min(a.n, s * a.Ls);
     const int len = (int)(min(a.n, (s + 1) * a.Ls) - start);
     const double W_N = st->W_N;
@@ -232,18 +236,11 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
     L.len = len;
     const uint64_t lt = lanemask_lt();
 
-    // All loads of the section in flight at once, then the stable partition:
-    // lights to the front in index order, heavies to the back (h[k] at
-    // len-1-k) in index order.
-    double x[PSA_MAX_LEN / 64];
-#pragma unroll
-    for (int u = 0; u < PSA_MAX_LEN / 64; ++u) {
-        const int k = (u << 6) + lane;
-        x[u] = (k < len) ? (double)w[start + k] : 0.0;
-    }
-    // vmcnt(0) on every path: without it the compiler keeps the loads
-    // pending on the skipped blocks below and waits later, when the
-    // registers are reused, for every row store the chains issued by then.
+    // The stable partition: lights to the front in index order, heavies to
+    // the back (h[k] at len-1-k) in index order.  vmcnt(0) on every path:
+    // without it the compiler keeps the loads pending on the skipped blocks
+    // below and waits later, when the registers are reused, for every row
+    // store the chains issued by then.
     __builtin_amdgcn_s_waitcnt(0x0F70);
     int nL = 0, nH = 0;
 #pragma unroll
@@ -268,6 +265,7 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
         }
     }
     wave_lds_sync();
+    after_partition();
 
     int iLo = 0, iHi = 0, jLo = 0, jHi = 0;
     if (a.psa_plus) {
@@ -388,9 +386,7 @@ __device__ __forceinline__ void psa_local(const BuildArgs& a, int64_t s, double*
     }
     const double aggL = agg[0], aggH = agg[1];
     const uint64_t aggC = (uint64_t)(uint32_t)mL | ((uint64_t)(uint32_t)mH << 32);
-    bool timeout = false;
-    if (lane == 0) block_publish(a, s, aggC, aggL, aggH, timeout);
-    if (__ballot(timeout) != 0ull && lane == 0) atomicOr(&st->error, ERR_LOOKBACK_TIMEOUT);
+    if (lane == 0) block_store(a, s, aggC, aggL, aggH);
 }
 
 // Partition pass 2, section s by one wave: the section prefix P(s) from the
