@@ -47,20 +47,21 @@ inline void section_layout(int64_t n, int64_t param, int64_t* G, int64_t* Ls) {
     *Ls = n / *G + 1;
 }
 
-// Items one pack wave stages in LDS.  Splits are evenly spaced in list items
-// (n' = ceil(nToSplit k / p)), so the 64 chains of a wave hold at most
-// 64 ceil(nToSplit / p) + 1 items; nToSplit <= n + 2, and p falls below p_req
-// only through the N/10 and R/10 clamps, which keep chains near 10 items.  A
-// wave whose items exceed the capacity reads the lists from HBM instead.
-constexpr int32_t kPackMaxItems = 4096;
+// Items of the LDS pool of one pack wave: the items of all its 64 chains
+// when they fit (splits are evenly spaced in list items, n' = ceil(nToSplit
+// k / p), so 64 chains hold at most 64 ceil(nToSplit / p) + 1 items, with
+// nToSplit <= n + 2 and chains near 10 items when p is clamped below p_req),
+// at most kPackPoolItems; a wave whose chains need more runs them in rounds.
+constexpr int32_t kPackPoolItems = 1024;
 inline int32_t pack_cap_items(int64_t n, int64_t p_req) {
     if (p_req < 1) p_req = 1;
     int64_t per = (n + 2 + p_req - 1) / p_req;
     if (per < 11) per = 11;
     int64_t cap = (64 * (per + 1) + 2 + 15) / 16 * 16;
-    return (int32_t)(cap < kPackMaxItems ? cap : kPackMaxItems);
+    return (int32_t)(cap < kPackPoolItems ? cap : kPackPoolItems);
 }
-inline size_t pack_lds_bytes(int32_t cap) { return (size_t)cap * 12; }
+// Weight (8 B), item (4 B) and recorded alias (4 B) per pool item.
+inline size_t pack_lds_bytes(int32_t cap) { return (size_t)cap * 16; }
 
 size_t psa_lds_bytes(int64_t Ls_max);
 int64_t resident_blocks(const void* f, int threads, size_t lds);

@@ -601,8 +601,11 @@ struct StagedLists {
 // path.  Read indices are clamped to the chain's own ranges (lights up to
 // max(iUpper - 1, prev.i), heavies up to jUpper); a clamped value is never
 // consumed.
-template <bool AOS, typename Lists>
-__device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, Split prev, Split cur, double W_N) {
+// With REC, a light's row is not stored: its alias goes to rec[i - rbase]
+// and the caller writes the light rows afterwards in list order.
+template <bool AOS, bool REC, typename Lists>
+__device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, Split prev, Split cur, double W_N,
+                                           int32_t* rec = nullptr, int rbase = 0) {
     const int iUpper = cur.i, jUpper = cur.j;
     int i = prev.i, j = prev.j;
     const int iLast = max(iUpper - 1, i);
@@ -628,7 +631,8 @@ __device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, S
         const int alias = ex ? hi1 : hj;
         const double next = ex ? hw1 : lw;
         w += next - W_N;
-        if ((uint32_t)item < (uint32_t)a.n) store_row<AOS>(a, item, rw, alias);
+        if (REC && !ex) rec[i - rbase] = alias;
+        else if ((uint32_t)item < (uint32_t)a.n) store_row<AOS>(a, item, rw, alias);
         j += ex ? 1 : 0;
         i += ex ? 0 : 1;
         hj = ex ? hi1 : hj;
@@ -841,10 +845,14 @@ __device__ __forceinline__ void split_bisect_block(const BuildArgs& a, int64_t b
     if (!ok) atomicAdd((unsigned long long*)&st->split_fallbacks, 1ull);
 }
 
-// Pack of chains k0+1 .. k0+64 (k0 = 64 blk) by one wave.  Their light items
-// [i_k0, i_kE) and heavy items [j_k0, j_kE] are two contiguous ranges, copied
-// to LDS with coalesced loads when they fit; otherwise the wave reads the
-// lists from HBM.
+// Pack of chains k0+1 .. k0+64 (k0 = 64 blk) by one wave, in rounds.  A
+// round takes the next chains whose lights [i_a, i_b) and heavies [j_a, j_b]
+// (contiguous list ranges) fit the LDS pool of a.pack_cap items, copies them
+// there with coalesced loads, runs the chains (heavy rows stored at once,
+// light aliases recorded in the pool) and then stores the light rows in list
+// order, where items rise, so that a store instruction covers neighbouring
+// rows.  Chains that do not fit alone, and waves with a non-monotone split
+// (a bisection fell back), read the lists from HBM.
 template <bool AOS>
 __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t blk, double* lds_raw) {
     const DevState* st = a.st;
@@ -853,29 +861,39 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
     const int lane = threadIdx.x;
     const int64_t k0 = blk * 64;
     if (k0 + 1 > p) return;
-    const int64_t kE = min(k0 + 64, p);
-    const Split lo = a.splits[k0], hi = a.splits[kE];
-    const int64_t k = k0 + 1 + lane;
+    const int nch = (int)(min(k0 + 64, p) - k0);
+    const bool mine = lane < nch;
     Split prev{0, 0, 0.0}, cur{0, 0, 0.0};
-    bool inside = true;
-    if (k <= kE) {
-        prev = a.splits[k - 1];
-        cur = a.splits[k];
-        inside = prev.i >= lo.i && prev.i <= cur.i && cur.i <= hi.i && prev.j >= lo.j && prev.j <= cur.j &&
-                 cur.j <= hi.j;
+    if (mine) {
+        prev = a.splits[k0 + lane];
+        cur = a.splits[k0 + lane + 1];
     }
-    const int nLu = hi.i - lo.i, nHu = hi.j - lo.j + 1;
-    // Stage only when every chain of the wave stays inside the copied ranges
-    // (always, unless a bisection fell back) and the ranges fit the pool:
-    // weights at sW, items at sI, lights at [0, nLu), heavies after them.
-    const bool staged = __ballot(!inside) == 0ull && nLu >= 0 && nHu >= 1 && nLu + nHu <= a.pack_cap;
+    const double W_N = st->W_N;
+    const GlobalLists gl{a.Li, a.Hi, a.Lw, a.Hw};
+    if (__ballot(mine && !(prev.i <= cur.i && prev.j <= cur.j)) != 0ull) {
+        if (mine) pack_chain<AOS, false>(a, gl, prev, cur, W_N);
+        return;
+    }
+    const int cap = a.pack_cap;
     double* sW = lds_raw;
-    int32_t* sI = reinterpret_cast<int32_t*>(sW + a.pack_cap);
-    if (staged) {
+    int32_t* sI = reinterpret_cast<int32_t*>(sW + cap);
+    int32_t* sA = sI + cap;
+    int ta = 0;
+    while (ta < nch) {
+        const int Ia = __shfl(prev.i, ta, 64), Ja = __shfl(prev.j, ta, 64);
+        const bool fits = mine && lane >= ta && (cur.i - Ia) + (cur.j - Ja + 1) <= cap;
+        const int tb = ta + __popcll(__ballot(fits));
+        if (tb == ta) {
+            if (lane == ta) pack_chain<AOS, false>(a, gl, prev, cur, W_N);
+            ta += 1;
+            continue;
+        }
+        const int Ib = __shfl(cur.i, tb - 1, 64), Jb = __shfl(cur.j, tb - 1, 64);
+        const int nLu = Ib - Ia, nHu = Jb - Ja + 1;
         for (int hv = 0; hv < 2; ++hv) {
             const int m = hv ? nHu : nLu, base = hv ? nLu : 0;
-            const double* gw = hv ? a.Hw + lo.j : a.Lw + lo.i;
-            const int32_t* gi = hv ? a.Hi + lo.j : a.Li + lo.i;
+            const double* gw = hv ? a.Hw + Ja : a.Lw + Ia;
+            const int32_t* gi = hv ? a.Hi + Ja : a.Li + Ia;
             for (int t0 = 0; t0 < m; t0 += 8 * 64) {
                 double wv[8];
                 int32_t iv[8];
@@ -892,15 +910,17 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
             }
         }
         wave_lds_sync();
-    }
-    if (k > kE) return;
-    const double W_N = st->W_N;
-    if (staged) {
-        const StagedLists l{sI, sI + nLu, sW, sW + nLu, lo.i, lo.j};
-        pack_chain<AOS>(a, l, prev, cur, W_N);
-    } else {
-        const GlobalLists l{a.Li, a.Hi, a.Lw, a.Hw};
-        pack_chain<AOS>(a, l, prev, cur, W_N);
+        if (lane >= ta && lane < tb) {
+            const StagedLists l{sI, sI + nLu, sW, sW + nLu, Ia, Ja};
+            pack_chain<AOS, true>(a, l, prev, cur, W_N, sA, Ia);
+        }
+        wave_lds_sync();
+        for (int r = lane; r < nLu; r += 64) {
+            const int it = sI[r];
+            if ((uint32_t)it < (uint32_t)a.n) store_row<AOS>(a, it, sW[r], sA[r]);
+        }
+        wave_lds_sync();
+        ta = tb;
     }
 }
 
