@@ -11,9 +11,10 @@ import numpy as np
 from . import _lib
 from ._lib import (ATG_AOS, ATG_F32, ATG_F64, ATG_MEM_DEVICE, ATG_MEM_HOST, ATG_SAMPLE_BASIC,
                    ATG_SAMPLE_SECTIONED, ATG_SOA, ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED, DIST, AtgError,
-                   BuildOpts, Stats, check, lib)
+                   BuildOpts, Stats, TableFileInfo, check, lib)
 
-__all__ = ["Table", "Batch", "autotune", "generate_weights", "generate_batch", "default_opts", "lib", "AtgError", "DIST"]
+__all__ = ["Table", "Batch", "autotune", "generate_weights", "generate_batch", "default_opts", "lib", "AtgError", "DIST",
+           "write_table_file", "read_table_file"]
 
 
 def _ptr(a):
@@ -81,6 +82,32 @@ def autotune(w, dtype="f64", layout="aos", psa_plus=True, split="bisect", learne
                              C.byref(ms)))
     return {"section_size": int(best.section_size), "chains": int(best.chains), "num_splits": int(best.num_splits),
             "split": split, "psa_plus": psa_plus}, ms.value
+
+
+def write_table_file(path, info, w, alias):
+    """Writes a table file (atg_table_file_write): info is a dict with n, W and
+    optionally the build shape (layout, source_dtype, psa_plus, chains,
+    split_method, section_size, num_splits) and seed."""
+    fi = TableFileInfo()
+    for k, _ in TableFileInfo._fields_:
+        if k in info and k not in ("checksum", "reserved"):
+            setattr(fi, k, info[k])
+    wa = np.ascontiguousarray(w, dtype=np.float64)
+    aa = np.ascontiguousarray(alias, dtype=np.int32)
+    assert wa.size == aa.size == fi.n
+    check(lib().atg_table_file_write(str(path).encode(), C.byref(fi), _ptr(wa), _ptr(aa)))
+
+
+def read_table_file(path, rows=True):
+    """(info dict, row weights, aliases) of a table file; rows=False reads the header only."""
+    fi = TableFileInfo()
+    check(lib().atg_table_file_read(str(path).encode(), C.byref(fi), None, None, 0))
+    if not rows:
+        return fi.as_dict(), None, None
+    w = np.empty(fi.n, np.float64)
+    a = np.empty(fi.n, np.int32)
+    check(lib().atg_table_file_read(str(path).encode(), C.byref(fi), _ptr(w), _ptr(a), fi.n))
+    return fi.as_dict(), w, a
 
 
 class Table:
@@ -155,6 +182,21 @@ class Table:
         a = np.empty(self.n, dtype=np.int32)
         check(lib().atg_get_table(self.h, _ptr(w), _ptr(a), ATG_MEM_HOST, None))
         return w, a
+
+    def save(self, path, seed=0):
+        """Writes the built table to a table file (atg_save_table)."""
+        check(lib().atg_save_table(self.h, str(path).encode(), int(seed)))
+
+    def set_table(self, w, alias, W, stream=None):
+        """Loads rows (e.g. from read_table_file) for sampling (atg_set_table)."""
+        if hasattr(w, "is_cuda") and w.is_cuda:
+            self._keep = (w, alias)
+            check(lib().atg_set_table(self.h, _ptr(w), _ptr(alias), float(W), ATG_MEM_DEVICE, _stream(stream)))
+        else:
+            wa = np.ascontiguousarray(w, dtype=np.float64)
+            aa = np.ascontiguousarray(alias, dtype=np.int32)
+            assert wa.size == aa.size == self.n
+            check(lib().atg_set_table(self.h, _ptr(wa), _ptr(aa), float(W), ATG_MEM_HOST, _stream(stream)))
 
     def device_table_ptr(self):
         p0, p1 = C.c_void_p(), C.c_void_p()
@@ -256,6 +298,10 @@ class Batch:
 
     def build(self, stream=None):
         check(lib().atg_batch_build(self.h, _stream(stream)))
+
+    def use_graph(self, enable=True):
+        """Captures the build into a hipGraph replayed by build() (atg_batch_use_graph)."""
+        check(lib().atg_batch_use_graph(self.h, 1 if enable else 0))
 
     def sync(self, stream=None):
         check(lib().atg_batch_sync(self.h, _stream(stream)))

@@ -9,9 +9,9 @@ import os
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "libatg.so")
 
-ATG_OK, ATG_EINVAL, ATG_ENOMEM, ATG_EDEVICE, ATG_ESTATE, ATG_ENOTSUP, ATG_ETIMEOUT = range(7)
+ATG_OK, ATG_EINVAL, ATG_ENOMEM, ATG_EDEVICE, ATG_ESTATE, ATG_ENOTSUP, ATG_ETIMEOUT, ATG_EIO = range(8)
 STATUS_NAMES = {0: "ATG_OK", 1: "ATG_EINVAL", 2: "ATG_ENOMEM", 3: "ATG_EDEVICE", 4: "ATG_ESTATE",
-                5: "ATG_ENOTSUP", 6: "ATG_ETIMEOUT"}
+                5: "ATG_ENOTSUP", 6: "ATG_ETIMEOUT", 7: "ATG_EIO"}
 ATG_F64, ATG_F32 = 0, 1
 ATG_AOS, ATG_SOA = 0, 1
 ATG_MEM_HOST, ATG_MEM_DEVICE = 0, 1
@@ -41,6 +41,16 @@ class Stats(C.Structure):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "phase_ms"}
         d["phase_ms"] = list(self.phase_ms)
         return d
+
+
+class TableFileInfo(C.Structure):
+    _fields_ = [("n", C.c_int64), ("W", C.c_double), ("layout", C.c_int32), ("source_dtype", C.c_int32),
+                ("psa_plus", C.c_int32), ("chains", C.c_int32), ("split_method", C.c_int32),
+                ("reserved", C.c_int32), ("section_size", C.c_int64), ("num_splits", C.c_int64),
+                ("seed", C.c_uint64), ("checksum", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 # (name, restype, argtypes) for every entry point of include/atg.h
@@ -75,6 +85,11 @@ SIGNATURES = [
     ("atg_batch_get_tables", _I32, [_P, _P, _P, _P]),
     ("atg_batch_sample", _I32, [_P, _U64, _U64, _P, _P]),
     ("atg_batch_latencies", _I32, [_P, _P]),
+    ("atg_batch_use_graph", _I32, [_P, _I32]),
+    ("atg_table_file_write", _I32, [C.c_char_p, C.POINTER(TableFileInfo), _P, _P]),
+    ("atg_table_file_read", _I32, [C.c_char_p, C.POINTER(TableFileInfo), _P, _P, _I64]),
+    ("atg_save_table", _I32, [_P, C.c_char_p, _U64]),
+    ("atg_set_table", _I32, [_P, _P, _P, _D, _I32, _P]),
     ("atg_last_error", C.c_char_p, []),
     ("atg_version", C.c_char_p, []),
     ("atg_device_count", _I32, []),

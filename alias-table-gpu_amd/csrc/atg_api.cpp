@@ -372,6 +372,59 @@ atg_status atg_get_table(atg_table* t, double* w, int32_t* al, atg_mem mem, void
     return ATG_OK;
 }
 
+atg_status atg_save_table(atg_table* t, const char* path, uint64_t seed) {
+    if (!t || !path) return fail(ATG_EINVAL, "atg_save_table: NULL argument");
+    atg_stats st{};
+    atg_status r = atg_get_stats(t, &st);
+    if (r != ATG_OK) return r;
+    std::vector<double> w((size_t)t->n);
+    std::vector<int32_t> al((size_t)t->n);
+    r = atg_get_table(t, w.data(), al.data(), ATG_MEM_HOST, nullptr);
+    if (r != ATG_OK) return r;
+    atg_table_file_info info{};
+    info.n = t->n;
+    info.W = st.W;
+    info.layout = t->layout;
+    info.source_dtype = t->dtype;
+    info.psa_plus = t->opts.psa_plus;
+    info.chains = t->opts.chains;
+    info.split_method = t->opts.split_method;
+    info.section_size = t->opts.section_size;
+    info.num_splits = st.p;
+    info.seed = seed;
+    return atg_table_file_write(path, &info, w.data(), al.data());
+}
+
+atg_status atg_set_table(atg_table* t, const double* w, const int32_t* al, double W, atg_mem mem, void* stream) {
+    if (!t || !w || !al) return fail(ATG_EINVAL, "atg_set_table: NULL argument");
+    if (!(W > 0.0) || !std::isfinite(W)) return fail(ATG_EINVAL, "atg_set_table: W=%g", W);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = (size_t)t->n;
+    if (t->layout == ATG_SOA) {
+        const hipMemcpyKind k = mem == ATG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        HIPCHK(hipMemcpyAsync(t->d_tw, w, n * 8, k, s));
+        HIPCHK(hipMemcpyAsync(t->d_ta, al, n * 4, k, s));
+    } else if (mem == ATG_MEM_DEVICE) {
+        HIPCHK(launch_join_rows(w, al, t->d_rows, t->n, s));
+    } else {
+        std::vector<Row> rows(n);
+        for (size_t i = 0; i < n; ++i) rows[i] = Row{w[i], al[i], 0};
+        HIPCHK(hipMemcpyAsync(t->d_rows, rows.data(), n * sizeof(Row), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    // Sampling reads W/N from the device state; the rest describes no build.
+    DevState st{};
+    st.W = W;
+    st.W_N = W / (double)t->n;
+    st.cert_max_dL = 0ull;
+    st.cert_min_dH = 0x7FF0000000000000ull;
+    HIPCHK(hipMemcpyAsync(t->d_state, &st, sizeof(DevState), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    t->built = true;
+    t->rej_built = false;
+    return ATG_OK;
+}
+
 atg_status atg_table_device_ptr(atg_table* t, void** a, void** b) {
     if (!t || !a) return fail(ATG_EINVAL, "atg_table_device_ptr: NULL argument");
     if (t->layout == ATG_AOS) {

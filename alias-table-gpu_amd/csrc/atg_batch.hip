@@ -245,13 +245,27 @@ struct atg_batch_s {
     int32_t* d_maptab = nullptr;        // block->table maps of all phases
     int64_t* d_mapbase = nullptr;       // first block of every table, all phases
     bool weights_set = false, built = false;
+    // The build's launch sequence captured once and replayed (atg_batch_use_graph).
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
 
 inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
+void drop_graph(atg_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->cap_stream = nullptr;
+}
+
 void free_batch(atg_batch* b) {
+    drop_graph(b);
     void* ptrs[] = {b->d_weights_owned, b->d_args, b->d_offsets, b->d_ticks, b->d_state, b->d_rows,
                     b->d_dpool, b->d_slots, b->d_ipool, b->d_splits, b->d_maptab, b->d_mapbase};
     for (void* p : ptrs)
@@ -517,8 +531,36 @@ atg_status atg_batch_build(atg_batch* b, void* stream) {
     if (!b) return fail(ATG_EINVAL, "atg_batch_build: NULL batch");
     if (!b->weights_set) return fail(ATG_ESTATE, "atg_batch_build: weights not set");
     hipStream_t s = (hipStream_t)stream;
-    HIPCHK(b->dtype == ATG_F32 ? launch_batch<float>(b, s) : launch_batch<double>(b, s));
+    if (b->exec) HIPCHK(hipGraphLaunch(b->exec, s));
+    else HIPCHK(b->dtype == ATG_F32 ? launch_batch<float>(b, s) : launch_batch<double>(b, s));
     b->built = true;
+    return ATG_OK;
+}
+
+// Captures the build's launch sequence (kernels only: no copies or memsets)
+// on a private stream into a graph that atg_batch_build replays from then on;
+// enable = 0 drops the graph.  The kernels read every table's arguments from
+// device memory, which atg_batch_set_weights rewrites, so the graph stays
+// valid across new weights.
+atg_status atg_batch_use_graph(atg_batch* b, int32_t enable) {
+    if (!b) return fail(ATG_EINVAL, "atg_batch_use_graph: NULL batch");
+    drop_graph(b);
+    if (!enable) return ATG_OK;
+    HIPCHK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = b->dtype == ATG_F32 ? launch_batch<float>(b, b->cap_stream) : launch_batch<double>(b, b->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->cap_stream, &g);
+    b->graph = g;
+    if (e != hipSuccess || e2 != hipSuccess) {
+        drop_graph(b);
+        return fail(ATG_EDEVICE, "atg_batch_use_graph: capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    const hipError_t e3 = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) {
+        drop_graph(b);
+        return fail(ATG_EDEVICE, "atg_batch_use_graph: instantiate failed: %s", hipGetErrorString(e3));
+    }
     return ATG_OK;
 }
 

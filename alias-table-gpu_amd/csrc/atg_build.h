@@ -74,5 +74,6 @@ hipError_t launch_split(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split_learned(const BuildArgs& a, hipStream_t s);
 hipError_t launch_pack(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split_rows(const Row* rows, double* w, int32_t* al, int64_t n, hipStream_t s);
+hipError_t launch_join_rows(const double* w, const int32_t* al, Row* rows, int64_t n, hipStream_t s);
 
 }  // namespace atg

@@ -181,6 +181,23 @@ hipError_t launch_split_rows(const Row* rows, double* w, int32_t* al, int64_t n,
     return hipGetLastError();
 }
 
+// Separate weight and alias arrays -> AoS rows (a table loaded for sampling).
+__global__ __launch_bounds__(256) void join_rows(const double* w, const int32_t* al, Row* rows, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        Row r;
+        r.weight = w[i];
+        r.alias = al[i];
+        r.pad = 0;
+        *reinterpret_cast<uint4*>(&rows[i]) = *reinterpret_cast<const uint4*>(&r);
+    }
+}
+
+hipError_t launch_join_rows(const double* w, const int32_t* al, Row* rows, int64_t n, hipStream_t s) {
+    const int64_t g = std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(join_rows, dim3(g > 0 ? g : 1), dim3(256), 0, s, w, al, rows, n);
+    return hipGetLastError();
+}
+
 hipError_t set_partition_lds_limit(size_t bytes) {
     const void* fns[] = {(const void*)partition_local<double, true>, (const void*)partition_local<double, false>,
                          (const void*)partition_local<float, true>, (const void*)partition_local<float, false>};

@@ -26,9 +26,10 @@ METRIC = "build Gitems/s + Gsamples/s at N=1e8 (%HBM peak); learned/batched vs p
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per build phase measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (calibrated on a 1 GiB device copy) for the headline workload at this shape;
-# profiles/build_pareto1e8_s1536_c16_hbm_traffic.md has the commands and counts.
-MEASURED_TRAFFIC = {(100_000_000, 1536, 16, 2631578): {"sum": 804_000_000, "partition": 4_250_000_000,
-                                                       "split": 4_872_000_000, "pack": 1_853_000_000}}
+# profiles/build_pareto1e8_s2048_c24_p1562500_hbm_traffic.md has the commands
+# and counts.
+MEASURED_TRAFFIC = {(100_000_000, 2048, 24, 1562500): {"sum": 803_000_000, "partition": 4_232_000_000,
+                                                       "split": 3_004_000_000, "pack": 1_822_000_000}}
 
 
 def log(*a):

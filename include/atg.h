@@ -30,7 +30,8 @@ typedef enum {
     ATG_EDEVICE = 3,  /* HIP runtime error */
     ATG_ESTATE = 4,   /* call out of order (e.g. sample before build) */
     ATG_ENOTSUP = 5,  /* option not supported */
-    ATG_ETIMEOUT = 6  /* a device-side bounded wait expired (internal error) */
+    ATG_ETIMEOUT = 6, /* a device-side bounded wait expired (internal error) */
+    ATG_EIO = 7       /* file I/O failed, or a table file is damaged (bad magic, checksum) */
 } atg_status;
 
 typedef enum { ATG_F64 = 0, ATG_F32 = 1 } atg_dtype;     /* weight type; sums are always FP64 */
@@ -139,6 +140,43 @@ atg_status atg_get_splits(atg_table* t, int32_t* i, int32_t* j, double* spill, i
 atg_status atg_get_lh(atg_table* t, int32_t* L_item, double* L_w, double* L_pref, int32_t* H_item,
                       double* H_w, double* H_pref);
 
+/* ---- table files ------------------------------------------------------ */
+
+/* Header of a versioned table file (no counterpart in the reference, whose
+ * tables live in memory only: AliasTable.cuh:13-24, postBuild at
+ * AliasTableSplitGpu.cu:599-612).  The file holds the header, the n row
+ * weights (FP64) and the n aliases (int32); checksums guard both. */
+typedef struct {
+    int64_t n;
+    double W;                /* canonical FP64 sum of the weights */
+    int32_t layout;          /* atg_layout the table was built with */
+    int32_t source_dtype;    /* atg_dtype of the weights it was built from */
+    int32_t psa_plus;        /* build shape: PSA+, chains, section size, split method, effective p */
+    int32_t chains;
+    int32_t split_method;
+    int32_t reserved;        /* 0 */
+    int64_t section_size;
+    int64_t num_splits;
+    uint64_t seed;           /* the caller's tag, e.g. the generator seed of the weights; 0 if none */
+    uint64_t checksum;       /* read: the rows' checksum stored in the file; ignored on write */
+} atg_table_file_info;
+
+/* Host-only: writes a table (weights and aliases of info->n rows) to path. */
+atg_status atg_table_file_write(const char* path, const atg_table_file_info* info, const double* weights,
+                                const int32_t* aliases);
+/* Host-only: reads the header into *info and, when weights and aliases are
+ * not NULL (capacity >= n entries each), the rows; verifies both checksums. */
+atg_status atg_table_file_read(const char* path, atg_table_file_info* info, double* weights, int32_t* aliases,
+                               int64_t capacity);
+/* Copies a built table out (atg_get_table) and writes it with its build
+ * shape and W. */
+atg_status atg_save_table(atg_table* t, const char* path, uint64_t seed);
+/* Loads rows into a created table of the same n for sampling (for a table
+ * read from a file); W is the weight sum the rows were built for.  The table
+ * counts as built until the next atg_build or atg_set_weights. */
+atg_status atg_set_table(atg_table* t, const double* weights, const int32_t* aliases, double W, atg_mem mem,
+                         void* stream);
+
 /* ---- sampling: Sampler::benchmarkSampling / getSamplingDistribution ---- */
 
 /* SamplerGpuBasic / SamplerGpuSectioned executeBenchmarkSampling
@@ -202,6 +240,12 @@ atg_status atg_batch_build(atg_batch* b, void* stream);
 atg_status atg_batch_sync(atg_batch* b, void* stream);
 /* Rows (AoS, 16 B) of all tables, concatenated; per-table W. */
 atg_status atg_batch_get_tables(atg_batch* b, double* weights, int32_t* aliases, double* W_per_table);
+/* enable = 1: captures the build's launch sequence (kernels only) into a
+ * hipGraph that atg_batch_build replays from then on; 0: back to direct
+ * launches.  The graph stays valid across atg_batch_set_weights.  Replaces
+ * the reference's per-phase launches of one table (AliasTableSplitGpu.cu:388-468)
+ * for many tables at once. */
+atg_status atg_batch_use_graph(atg_batch* b, int32_t enable);
 /* `per_table` samples from every table; out[t * per_table + k] = item index
  * local to table t. */
 atg_status atg_batch_sample(atg_batch* b, uint64_t per_table, uint64_t seed, int32_t* out,

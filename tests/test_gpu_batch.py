@@ -136,3 +136,47 @@ def test_batch_bad_table_reported(gpu):
     with pytest.raises(atg.AtgError) as e:
         b.sync()
     assert "table 1" in str(e.value)
+
+
+def test_batch_graph_replays_match_direct_launches(gpu, oracle):
+    """The build captured in a hipGraph (atg_batch_use_graph), replayed several
+    times with other device work and synchronisation between the replays and
+    with new weights, gives the tables of direct launches every time."""
+    torch = gpu
+    sizes = [5000, 123, 40000, 64, 1, 70000]
+    offsets, w = make_batch(sizes, "pareto")
+    direct = atg.Batch(offsets)
+    direct.set_weights(w)
+    direct.build()
+    direct.sync()
+    ref = direct.tables()
+    b = atg.Batch(offsets)
+    b.set_weights(w)
+    b.use_graph(True)
+    for rep in range(4):
+        x = torch.randn(1 << 20, device="cuda")
+        (x * 2).sum().item()
+        b.build()
+        b.sync()
+        got = b.tables()
+        assert np.array_equal(bits(got[0]), bits(ref[0])) and np.array_equal(got[1], ref[1]), rep
+        assert np.array_equal(bits(got[2]), bits(ref[2])), rep
+    assert np.all(b.latencies_ns() > 0)
+    # New weights: the same graph builds the new tables.
+    _, w2 = make_batch(sizes, "exponential", seed=99)
+    b.set_weights(w2)
+    b.build()
+    b.sync()
+    rw, ra, W = b.tables()
+    p_auto = 320 * torch.cuda.get_device_properties(0).multi_processor_count
+    for t in range(len(sizes)):
+        lo, hi = offsets[t], offsets[t + 1]
+        r = oracle.build(w2[lo:hi], num_splits=p_auto)
+        assert np.array_equal(bits(rw[lo:hi]), bits(r["row_w"])) and np.array_equal(ra[lo:hi], r["row_alias"]), t
+    # Back to direct launches.
+    b.use_graph(False)
+    b.set_weights(w)
+    b.build()
+    b.sync()
+    got = b.tables()
+    assert np.array_equal(bits(got[0]), bits(ref[0])) and np.array_equal(got[1], ref[1])
