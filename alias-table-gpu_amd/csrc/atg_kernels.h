@@ -23,24 +23,25 @@ __device__ __forceinline__ void wave_lds_sync() {
 // ---------------------------------------------------------------------------
 // 1. Section sums.  Lane l of the wave that owns a section adds items
 //    l, l+64, ... in order; the lane sums meet in the pairwise tree.  A NaN,
-//    infinite or negative weight turns the section sum into NaN.
+//    infinite or negative weight turns the section sum into NaN.  The loads
+//    are non-temporal: the pass reads every weight once.
 template <typename T>
 __device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t end, int lane) {
     double acc = 0.0;
     bool bad = false;
     int64_t k = start + lane;
-    for (; k + 15 * 64 < end; k += 16 * 64) {
-        double x[16];
+    for (; k + 31 * 64 < end; k += 32 * 64) {
+        double x[32];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) x[u] = (double)w[k + u * 64];
+        for (int u = 0; u < 32; ++u) x[u] = (double)__builtin_nontemporal_load(&w[k + u * 64]);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
+        for (int u = 0; u < 32; ++u) {
             bad |= !(x[u] >= 0.0) || (x[u] == __builtin_inf());
             acc = acc + x[u];
         }
     }
     for (; k < end; k += 64) {
-        const double x = (double)w[k];
+        const double x = (double)__builtin_nontemporal_load(&w[k]);
         bad |= !(x >= 0.0) || (x == __builtin_inf());
         acc = acc + x;
     }
