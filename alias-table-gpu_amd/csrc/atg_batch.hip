@@ -94,7 +94,7 @@ template <typename T>
 __global__ __launch_bounds__(64) void seg_local(SegArgs g, SegMap m) {
     extern __shared__ double lds_raw[];
     const int lane = (int)threadIdx.x;
-    double x[PSA_MAX_LEN / 64];
+    T x[PSA_MAX_LEN / 64];
     auto load = [&](int64_t v) {
         const int64_t t = m.tab[v];
         const BuildArgs& a = g.tables[t];
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(64) void seg_local(SegArgs g, SegMap m) {
             next();
             continue;
         }
-        psa_section<true>(a, v - m.base[t], x, lds_raw, lane, next);
+        psa_section<T, true>(a, v - m.base[t], x, lds_raw, lane, next);
     }
 }
 

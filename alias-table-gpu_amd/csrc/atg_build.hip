@@ -43,12 +43,12 @@ __global__ __launch_bounds__(64) void partition_local(BuildArgs a) {
     extern __shared__ double lds_raw[];
     if (a.st->error) return;
     const int lane = (int)threadIdx.x;
-    double x[PSA_MAX_LEN / 64];
+    T x[PSA_MAX_LEN / 64];
     int64_t s = blockIdx.x;
     if (s < a.G) section_load<T>(a, s, lane, x);
     for (; s < a.G; s += gridDim.x) {
         const int64_t sn = s + gridDim.x;
-        psa_section<AOS>(a, s, x, lds_raw, lane, [&]() {
+        psa_section<T, AOS>(a, s, x, lds_raw, lane, [&]() {
             if (sn < a.G) section_load<T>(a, sn, lane, x);
         });
     }

@@ -29,21 +29,23 @@ template <typename T>
 __device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t end, int lane) {
     double acc = 0.0;
     bool bad = false;
-    int64_t k = start + lane;
-    for (; k + 31 * 64 < end; k += 32 * 64) {
-        double x[32];
-#pragma unroll
-        for (int u = 0; u < 32; ++u) x[u] = (double)__builtin_nontemporal_load(&w[k + u * 64]);
+    for (int64_t k0 = start; k0 < end; k0 += 32 * 64) {
+        // 32 items per lane in flight, kept raw: a conversion next to each
+        // load would serialise the loads.
+        T xr[32];
 #pragma unroll
         for (int u = 0; u < 32; ++u) {
-            bad |= !(x[u] >= 0.0) || (x[u] == __builtin_inf());
-            acc = acc + x[u];
+            const int64_t k = k0 + u * 64 + lane;
+            xr[u] = (k < end) ? __builtin_nontemporal_load(&w[k]) : T(0);
         }
-    }
-    for (; k < end; k += 64) {
-        const double x = (double)__builtin_nontemporal_load(&w[k]);
-        bad |= !(x >= 0.0) || (x == __builtin_inf());
-        acc = acc + x;
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            if (k0 + u * 64 + lane < end) {
+                const double x = (double)xr[u];
+                bad |= !(x >= 0.0) || (x == __builtin_inf());
+                acc = acc + x;
+            }
+        }
     }
     double sum = wave_tree_sum(acc);
     if (__ballot(bad) != 0ull) sum = __builtin_nan("");
@@ -203,17 +205,18 @@ __device__ __forceinline__ void chain_positions(int r0, int off, int lo, int* po
     for (int u = 0; u < U; ++u) pos[u] = b[u] + (r0 + 64 * u - c[u]);
 }
 
-// The weights of section s, lane-strided (item 64 u + lane in x[u]), as
-// FP64; all loads in flight at once.
+// The weights of section s, lane-strided (item 64 u + lane in x[u]), in
+// their own type (converted to FP64 where they are used, so that no
+// conversion waits for its load); all loads in flight at once.
 template <typename T>
-__device__ __forceinline__ void section_load(const BuildArgs& a, int64_t s, int lane, double (&x)[PSA_MAX_LEN / 64]) {
+__device__ __forceinline__ void section_load(const BuildArgs& a, int64_t s, int lane, T (&x)[PSA_MAX_LEN / 64]) {
     const T* w = (const T*)a.weights;
     const int64_t start = min(a.n, s * a.Ls);
     const int len = (int)(min(a.n, (s + 1) * a.Ls) - start);
 #pragma unroll
     for (int u = 0; u < PSA_MAX_LEN / 64; ++u) {
         const int k = (u << 6) + lane;
-        x[u] = (k < len) ? (double)w[start + k] : 0.0;
+        x[u] = (k < len) ? w[start + k] : T(0);
     }
 }
 
@@ -224,8 +227,8 @@ __device__ __forceinline__ void section_load(const BuildArgs& a, int64_t s, int 
 // aggregates (stored as the level-0 block of the dyadic section tree) and the
 // leftovers staged in rank order at the section's own range (lights, then
 // heavies).  Nothing here waits for another section's work.
-template <bool AOS, typename AfterPartition>
-__device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, double (&x)[PSA_MAX_LEN / 64],
+template <typename T, bool AOS, typename AfterPartition>
+__device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x)[PSA_MAX_LEN / 64],
                                             double* lds_raw, int lane, AfterPartition after_partition) {
     DevState* st = a.st;
     const int64_t start = min(a.n, s * a.Ls);
@@ -249,16 +252,17 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, doubl
         const int k = (u << 6) + lane;
         if ((u << 6) < len) {
             const bool valid = k < len;
-            const bool light = valid && x[u] <= W_N, heavy = valid && !(x[u] <= W_N);
+            const double xv = (double)x[u];
+            const bool light = valid && xv <= W_N, heavy = valid && !(xv <= W_N);
             const uint64_t bl = __ballot(light), bh = __ballot(heavy);
             if (light) {
                 const int pos = nL + __popcll(bl & lt);
-                L.Lw(pos) = x[u];
+                L.Lw(pos) = xv;
                 L.Li(pos) = (uint16_t)k;
             }
             if (heavy) {
                 const int pos = nH + __popcll(bh & lt);
-                L.Hw(pos) = x[u];
+                L.Hw(pos) = xv;
                 L.Hi(pos) = (uint16_t)k;
             }
             nL += __popcll(bl);
@@ -307,6 +311,31 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, doubl
                 const bool ex = wc <= thr;
                 const int room = ex ? jHi - 1 - j : iHi - 1 - i;
                 if (room <= 0) break;
+                // While every chain still running takes light steps (its heavy
+                // is not exhausted and a light follows), a short loop does
+                // exactly those steps: the row of the light, the residual
+                // update, the next light.  Chains whose lights outnumber the
+                // heavies' excess spend most of their steps here.
+                const uint64_t running = __ballot(true);
+                if (__ballot(!ex) == running) {
+                    // Lights i + 1 (nlw, nli) and i + 2 (pw, pi) are in flight.
+                    int q2 = min(i + 2, iHi - 1);
+                    double pw = L.w[q2];
+                    int pi = L.idx[q2];
+                    do {
+                        store_row<AOS>(a, start + li, lw, (int32_t)(start + hj));
+                        wc += lw - W_N;
+                        i += 1;
+                        lw = nlw;
+                        li = nli;
+                        nlw = pw;
+                        nli = pi;
+                        q2 = min(i + 2, iHi - 1);
+                        pw = L.w[q2];
+                        pi = L.idx[q2];
+                    } while (__ballot(!(wc <= thr) && i < iHi - 1) == running);
+                    continue;
+                }
                 const int item = ex ? hj : li;
                 const double rw = ex ? wc : lw;
                 const int alias = ex ? hi1 : hj;
