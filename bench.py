@@ -209,7 +209,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-section", type=int, default=2048, help="section size of every batched table")
-    ap.add_argument("--batch-chains", type=int, default=16)
+    ap.add_argument("--batch-chains", type=int, default=32)
     ap.add_argument("--batch-items-per-split", type=int, default=38)
     ap.add_argument("--batch-learned", action="store_true",
                     help="also the 16,384-table FP32 Zipf batch with the learned index (2.4e9 items)")
