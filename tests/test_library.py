@@ -62,3 +62,14 @@ def test_generator_shapes():
 def test_generate_rejects_bad_distribution():
     w = np.empty(10)
     assert atg.lib().atg_generate_weights(9999, 10, 1, 0.0, w.ctypes.data_as(C.c_void_p)) == _lib.ATG_EINVAL
+
+
+def test_plots_harness_lists_the_reference_sweeps():
+    from atg import plots
+    assert {"build_soa", "thread_number", "split_variants", "with_transfer", "light_vs_heavy",
+            "sampler_method", "sampling_speed", "distribution"} <= set(plots.SWEEPS)
+    import subprocess, sys, os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-m", "atg.plots", "--help"], capture_output=True, text=True,
+                       cwd=os.path.join(root, "alias-table-gpu_amd"))
+    assert r.returncode == 0 and "build_soa" in r.stdout
