@@ -114,8 +114,9 @@ class Table:
     """One alias table on the GPU (AliasTableSplitGpu<TableStorage> of the reference)."""
 
     def __init__(self, n, dtype="f64", layout="aos", psa_plus=True, chains=16, section_size=0, num_splits=0,
-                 split="bisect", learned_segments=4096, record_phases=False):
+                 split="bisect", learned_segments=4096, record_phases=False, interleave=False):
         o = default_opts()
+        o.interleave = 1 if interleave else 0
         o.psa_plus = 1 if psa_plus else 0
         o.chains = int(chains)
         o.section_size = int(section_size)

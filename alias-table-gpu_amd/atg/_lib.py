@@ -28,7 +28,7 @@ DIST = {
 class BuildOpts(C.Structure):
     _fields_ = [("psa_plus", C.c_int32), ("chains", C.c_int32), ("section_size", C.c_int64),
                 ("num_splits", C.c_int64), ("split_method", C.c_int32), ("learned_segments", C.c_int32),
-                ("record_phases", C.c_int32), ("reserved", C.c_int32)]
+                ("record_phases", C.c_int32), ("interleave", C.c_int32)]
 
 
 class Stats(C.Structure):

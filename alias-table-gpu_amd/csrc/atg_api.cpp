@@ -88,6 +88,11 @@ struct atg_table_s {
     bool weights_set = false, built = false;
     hipEvent_t ev[6] = {};
     bool events = false;
+    // Split/pack interleave: the splits run in chunks on sp_stream, each pack
+    // chunk waits for its splits.
+    hipStream_t sp_stream = nullptr;
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_chunk[kInterleaveChunks] = {};
 
     BuildArgs args() const {
         BuildArgs a{};
@@ -134,6 +139,7 @@ void atg_default_opts(atg_build_opts* o) {
     o->split_method = ATG_SPLIT_BISECT;
     o->learned_segments = 4096;
     o->record_phases = 0;
+    o->interleave = 0;
 }
 
 const char* atg_last_error(void) { return g_last_error.c_str(); }
@@ -154,6 +160,12 @@ static void free_table(atg_table* t) {
         if (p) (void)hipFree(p);
     if (t->events)
         for (auto& e : t->ev) (void)hipEventDestroy(e);
+    if (t->sp_stream) {
+        (void)hipStreamSynchronize(t->sp_stream);
+        (void)hipStreamDestroy(t->sp_stream);
+        (void)hipEventDestroy(t->ev_fork);
+        for (auto& e : t->ev_chunk) (void)hipEventDestroy(e);
+    }
 }
 
 void atg_destroy(atg_table* t) {
@@ -233,6 +245,14 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
             if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(ATG_EDEVICE, "hipEventCreate"));
         t->events = true;
     }
+    if (o.interleave) {
+        if (hipStreamCreateWithFlags(&t->sp_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&t->ev_fork, hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(ATG_EDEVICE, "atg_create: split stream"));
+        for (auto& e : t->ev_chunk)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                return cleanup(fail(ATG_EDEVICE, "atg_create: split events"));
+    }
     *out = t;
     return ATG_OK;
 }
@@ -286,10 +306,53 @@ atg_status atg_build_phase(atg_table* t, int32_t phase, void* stream) {
     return ATG_OK;
 }
 
+// Split and pack interleaved (AliasTableSplitGpu.cu:220-262): the splits run
+// in kInterleaveChunks chunks of whole 256-split blocks on the table's own
+// stream while the caller's stream packs the chains of each finished chunk
+// (pack chunk c needs splits up to 256 * its last split block).  Returns
+// false when p is too small to chunk.
+static bool build_split_pack_interleaved(atg_table* t, const BuildArgs& a, hipStream_t s, hipError_t& err) {
+    const int64_t SB = split_blocks(a), PB = pack_blocks(a);
+    const int C = (int)std::min<int64_t>(kInterleaveChunks, SB);
+    if (C < 2) return false;
+    const bool learned = t->opts.split_method == ATG_SPLIT_LEARNED;
+    hipStream_t sp = t->sp_stream;
+    err = hipEventRecord(t->ev_fork, s);
+    if (err == hipSuccess) err = hipStreamWaitEvent(sp, t->ev_fork, 0);
+    if (err == hipSuccess) err = launch_degenerate(a, sp);
+    if (err == hipSuccess && learned) err = launch_split_knots(a, sp);
+    for (int c = 0; c < C && err == hipSuccess; ++c) {
+        const int64_t b0 = SB * c / C, b1 = SB * (c + 1) / C;
+        err = learned ? launch_split_learned_blocks(a, sp, b0, b1) : launch_split_blocks(a, sp, b0, b1);
+        if (err == hipSuccess) err = hipEventRecord(t->ev_chunk[c], sp);
+    }
+    for (int c = 0; c < C && err == hipSuccess; ++c) {
+        const int64_t b0 = 4 * (SB * c / C);
+        const int64_t b1 = c == C - 1 ? PB : std::min<int64_t>(PB, 4 * (SB * (c + 1) / C));
+        err = hipStreamWaitEvent(s, t->ev_chunk[c], 0);
+        if (err == hipSuccess) err = launch_pack_blocks(a, s, std::min(b0, PB), b1);
+    }
+    return true;
+}
+
 atg_status atg_build(atg_table* t, void* stream) {
     if (!t) return fail(ATG_EINVAL, "atg_build: NULL table");
     if (!t->weights_set) return fail(ATG_ESTATE, "atg_build: weights not set");
     hipStream_t s = (hipStream_t)stream;
+    if (t->sp_stream && !t->events) {
+        const BuildArgs a = t->args();
+        HIPCHK(launch_sum(a, s));
+        HIPCHK(launch_partition(a, s));
+        hipError_t err = hipSuccess;
+        if (build_split_pack_interleaved(t, a, s, err)) {
+            HIPCHK(err);
+            t->built = true;
+            return ATG_OK;
+        }
+        atg_status rc = atg_build_phase(t, ATG_PHASE_SPLIT, stream);
+        if (rc == ATG_OK) rc = atg_build_phase(t, ATG_PHASE_PACK, stream);
+        return rc;
+    }
     // Events follow TableSplitTimer's phases (utils/Timer.cu:62-77); the
     // prefix sums are fused into the partition, so that phase reads 0.
     static const int before[4] = {0, 1, 3, 4};  // event recorded before each phase

@@ -84,17 +84,17 @@ __global__ void degenerate_fill(BuildArgs a) {
 // ---------------------------------------------------------------------------
 // 3b. Split: bisection per k with the reference's predicate; 64 probes bound
 //     the loop (the reference spins forever when FOUND is unreachable).
-__global__ __launch_bounds__(256) void split_bisect(BuildArgs a) {
-    split_bisect_block(a, blockIdx.x);
+__global__ __launch_bounds__(256) void split_bisect(BuildArgs a, int64_t blk0) {
+    split_bisect_block(a, blk0 + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
 // 3c. Pack with LDS staging (the role of PackMethodBasicShared /
 //     PackMethodChunkedShared, PackMethodBasicShared.cu:3-68).
 template <bool AOS>
-__global__ __launch_bounds__(64) void pack_staged(BuildArgs a) {
+__global__ __launch_bounds__(64) void pack_staged(BuildArgs a, int64_t blk0) {
     extern __shared__ double lds_raw[];
-    pack_staged_block<AOS>(a, blockIdx.x, lds_raw);
+    pack_staged_block<AOS>(a, blk0 + blockIdx.x, lds_raw);
 }
 
 // ---------------------------------------------------------------------------
@@ -150,21 +150,30 @@ hipError_t launch_degenerate(const BuildArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_split(const BuildArgs& a, hipStream_t s) {
-    hipError_t e = launch_degenerate(a, s);
-    if (e != hipSuccess) return e;
-    const int64_t kmax = a.p_req;  // effective p <= p_req
-    if (kmax >= 2) hipLaunchKernelGGL(split_bisect, dim3(blocks_for(kmax - 1, 256)), dim3(256), 0, s, a);
+// Blocks of 256 splits (k = 1 .. p_req - 1) and of 64 pack chains (k = 1 .. p_req).
+int64_t split_blocks(const BuildArgs& a) { return a.p_req >= 2 ? blocks_for(a.p_req - 1, 256) : 0; }
+int64_t pack_blocks(const BuildArgs& a) { return blocks_for(a.p_req, 64); }
+
+hipError_t launch_split_blocks(const BuildArgs& a, hipStream_t s, int64_t b0, int64_t b1) {
+    if (b1 > b0) hipLaunchKernelGGL(split_bisect, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, a, b0);
     return hipGetLastError();
 }
 
-hipError_t launch_pack(const BuildArgs& a, hipStream_t s) {
-    const int64_t kmax = a.p_req;
+hipError_t launch_split(const BuildArgs& a, hipStream_t s) {
+    hipError_t e = launch_degenerate(a, s);
+    if (e != hipSuccess) return e;
+    return launch_split_blocks(a, s, 0, split_blocks(a));  // effective p <= p_req
+}
+
+hipError_t launch_pack_blocks(const BuildArgs& a, hipStream_t s, int64_t b0, int64_t b1) {
     const size_t lds = pack_lds_bytes(a.pack_cap);
-    if (a.rows) hipLaunchKernelGGL(pack_staged<true>, dim3(blocks_for(kmax, 64)), dim3(64), lds, s, a);
-    else hipLaunchKernelGGL(pack_staged<false>, dim3(blocks_for(kmax, 64)), dim3(64), lds, s, a);
+    if (b1 <= b0) return hipSuccess;
+    if (a.rows) hipLaunchKernelGGL(pack_staged<true>, dim3((unsigned)(b1 - b0)), dim3(64), lds, s, a, b0);
+    else hipLaunchKernelGGL(pack_staged<false>, dim3((unsigned)(b1 - b0)), dim3(64), lds, s, a, b0);
     return hipGetLastError();
 }
+
+hipError_t launch_pack(const BuildArgs& a, hipStream_t s) { return launch_pack_blocks(a, s, 0, pack_blocks(a)); }
 
 // AoS rows -> separate weight and alias arrays (device to device copy-out).
 __global__ __launch_bounds__(256) void split_rows(const Row* rows, double* w, int32_t* al, int64_t n) {

@@ -20,7 +20,8 @@ atg_status state_status(const DevState& st);
 // Sets the chain count and requested splits of a created table in place.
 atg_status table_set_shape(atg_table* t, int32_t chains, int64_t num_splits);
 
-constexpr int64_t kMaxSectionLen = 2048;  // items of one section staged in registers + LDS (PSA_MAX_LEN)
+constexpr int64_t kMaxSectionLen = 2048;
+constexpr int kInterleaveChunks = 8;  // split/pack interleave chunks  // items of one section staged in registers + LDS (PSA_MAX_LEN)
 constexpr int64_t kMaxN = 0x7FFFFFF0LL;
 
 template <typename T>

@@ -61,7 +61,9 @@ typedef struct {
     int32_t split_method;    /* atg_split_method */
     int32_t learned_segments;/* knots of the learned split index (default 4096) */
     int32_t record_phases;   /* 1: hipEvents around sum/partition/prefixsum/split/pack */
-    int32_t reserved;        /* 0 */
+    int32_t interleave;      /* 1: atg_build runs split and pack interleaved on two streams
+                                (performSplitAndPackInterleaved, AliasTableSplitGpu.cu:220-262);
+                                not with record_phases.  Same table either way. */
 } atg_build_opts;
 
 typedef struct {

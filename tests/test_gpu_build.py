@@ -185,3 +185,30 @@ def test_split_ratio_mode(gpu, oracle):
     t = gpu_build(w, num_splits=-38)
     ref = oracle.build(w, num_splits=n // 38)
     assert_same_build(t, ref, n)
+
+
+@pytest.mark.parametrize("n,dist,split,layout,p", [
+    (1 << 20, "pareto", "bisect", "aos", 1 << 14),
+    (1 << 20, "heavy_mix", "learned", "aos", 1 << 14),
+    (300_000, "exponential", "bisect", "soa", 0),
+    (5000, "pareto", "bisect", "aos", 0),  # too few splits to chunk: sequential
+])
+def test_interleaved_split_pack_identical(gpu, oracle, n, dist, split, layout, p):
+    """Split and pack interleaved on two streams (the reference's
+    performSplitAndPackInterleaved) build the oracle's table, rebuild after
+    other device work, and match the sequential build."""
+    torch = gpu
+    w = atg.generate_weights(dist, n, seed=13, param=1.1 if dist == "pareto" else 0.9)
+    kw = dict(chains=24, section_size=2048, num_splits=p, split=split, layout=layout)
+    ref = oracle.build(w, chains=24, section_size=2048, num_splits=p or 320 * torch.cuda.get_device_properties(0).multi_processor_count,
+                       want_lists=False)
+    t = atg.Table(n, interleave=True, **kw)
+    t.set_weights(w)
+    for rep in range(3):
+        x = torch.randn(1 << 18, device="cuda")
+        (x * 3).sum().item()
+        t.build()
+        t.sync()
+        tw, ta = t.table()
+        assert np.array_equal(bits(tw), bits(ref["row_w"])) and np.array_equal(ta, ref["row_alias"]), rep
+    assert t.stats()["p"] == ref["stats"]["p"]
