@@ -82,6 +82,7 @@ struct atg_table_s {
     RejEntry* d_A = nullptr;
     uint64_t A_cap = 0;
     uint64_t *d_rej_tot = nullptr, *d_rej_off = nullptr;
+    RejTask* d_rej_tasks = nullptr;
     bool rej_built = false;
     // host side
     std::vector<double> h_weights;  // host copy when weights were given on the host
@@ -155,7 +156,7 @@ static void free_table(atg_table* t) {
     void* ptrs[] = {t->d_weights_owned, t->d_state, t->d_section_sums, t->d_tree, t->d_slots, t->d_sec_cert, t->d_stage_w,
                     t->d_stage_i, t->d_Li, t->d_Hi,
                     t->d_Lw, t->d_Lp, t->d_Hw, t->d_Hp, t->d_rows, t->d_tw, t->d_ta, t->d_splits, t->d_knots,
-                    t->d_checksum, t->d_A, t->d_rej_tot, t->d_rej_off};
+                    t->d_checksum, t->d_A, t->d_rej_tot, t->d_rej_off, t->d_rej_tasks};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->events)
@@ -641,6 +642,10 @@ atg_status atg_rejection_build(atg_table* t, void* stream) {
         if (dalloc(&t->d_A, t->A_cap) != hipSuccess) { (void)hipGetLastError(); return fail(ATG_ENOMEM, "rejection A"); }
         if (dalloc(&t->d_rej_tot, nb) != hipSuccess) { (void)hipGetLastError(); return fail(ATG_ENOMEM, "rejection totals"); }
         if (dalloc(&t->d_rej_off, nb) != hipSuccess) { (void)hipGetLastError(); return fail(ATG_ENOMEM, "rejection offsets"); }
+        if (dalloc(&t->d_rej_tasks, rejection_task_cap(t->A_cap)) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(ATG_ENOMEM, "rejection tasks");
+        }
     }
     // W: the canonical sum (RejectionSampling.cu:36 uses the host sum).
     BuildArgs ba = t->args();
@@ -655,6 +660,8 @@ atg_status atg_rejection_build(atg_table* t, void* stream) {
     r.block_off = t->d_rej_off;
     r.A = t->d_A;
     r.A_cap = t->A_cap;
+    r.tasks = t->d_rej_tasks;
+    r.tasks_cap = rejection_task_cap(t->A_cap);
     HIPCHK(launch_rejection_build(r, s));
     t->rej_built = true;
     return ATG_OK;

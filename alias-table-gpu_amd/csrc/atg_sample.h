@@ -45,6 +45,8 @@ struct RejArgs {
     RejEntry* A;          // |A| <= A_cap slots
     uint64_t A_cap;
     unsigned long long* trials;  // nullable
+    RejTask* tasks;        // long runs cut into tasks (rejection_task_cap(A_cap) entries)
+    uint32_t tasks_cap;
 };
 
 hipError_t launch_sample_basic(const SampleArgs& a, hipStream_t s);
@@ -53,5 +55,6 @@ hipError_t set_sample_lds_limit(int bytes);
 hipError_t launch_rejection_build(const RejArgs& a, hipStream_t s);
 hipError_t launch_rejection_sample(const RejArgs& a, const SampleArgs& sa, hipStream_t s);
 int64_t rejection_blocks(int64_t n);
+uint32_t rejection_task_cap(uint64_t A_cap);
 
 }  // namespace atg

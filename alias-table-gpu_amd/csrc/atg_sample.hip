@@ -149,10 +149,14 @@ __global__ __launch_bounds__(256) void sample_sectioned(SampleArgs a) {
 // ---------------------------------------------------------------------------
 // Rejection structure (RejectionSampling.cu:26-56).  c_i = floor(N w_i / W) + 1
 // is scanned with integers (exact in any order): per-block totals, one block
-// scanning the totals, then each block writes its items' slots of A.
+// scanning the totals, then each block writes its items' slots of A; the
+// slots of long runs are cut into tasks that the whole grid writes.
 #define REJ_ITEMS 4096
 #define REJ_THREADS 256
-#define REJ_SHORT 8  // runs longer than this are written by the whole block
+#define REJ_SHORT 8     // runs up to this are written by their lane
+#define REJ_WAVE 64     // runs up to this by their wave, longer ones as tasks
+#define REJ_TASK 4096   // slots per task
+#define REJ_LONG_BLOCKS 1024
 
 template <typename T>
 __device__ __forceinline__ uint32_t rej_count(const RejArgs& a, int64_t i, double W) {
@@ -160,13 +164,27 @@ __device__ __forceinline__ uint32_t rej_count(const RejArgs& a, int64_t i, doubl
     return (uint32_t)floor(x) + 1u;
 }
 
+// Per-block totals of the counts.  The block's 16 loads per thread are issued
+// together, raw: a conversion next to each load would serialise the loads.
 template <typename T>
 __global__ __launch_bounds__(REJ_THREADS) void rej_block_totals(RejArgs a) {
+    constexpr int U = REJ_ITEMS / REJ_THREADS;
     const double W = a.st->W;
+    const T* w = (const T*)a.weights;
     const int64_t base = (int64_t)blockIdx.x * REJ_ITEMS;
+    const int cnt = (int)min<int64_t>(REJ_ITEMS, a.n - base);
+    T wr[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int k = u * REJ_THREADS + threadIdx.x;
+        wr[u] = (k < cnt) ? w[base + k] : T(0);
+    }
     uint64_t acc = 0;
-    for (int64_t i = base + threadIdx.x; i < min(a.n, base + REJ_ITEMS); i += REJ_THREADS)
-        acc += rej_count<T>(a, i, W);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const double x = ((double)a.n * (double)wr[u]) / W;
+        acc += (u * REJ_THREADS + (int)threadIdx.x < cnt) ? (uint64_t)((uint32_t)floor(x) + 1u) : 0ull;
+    }
     acc = wave_sum_int<uint64_t>(acc);
     __shared__ uint64_t part[REJ_THREADS / 64];
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
@@ -178,106 +196,144 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_block_totals(RejArgs a) {
     }
 }
 
+// Single block: exclusive scan of the nb block totals (integers, exact in any
+// order).  Thread t owns the consecutive run [t q, t q + q) with q = ceil(nb /
+// 1024): a sequential pass over its run, one block scan of the run totals,
+// a second pass writing the offsets.
 __global__ __launch_bounds__(1024) void rej_scan_totals(RejArgs a) {
-    // Single block: exclusive scan of nb block totals (integers).
-    __shared__ uint64_t carry_sh;
-    if (threadIdx.x == 0) carry_sh = 0;
-    __syncthreads();
-    for (int64_t c0 = 0; c0 < a.nb; c0 += 1024) {
-        const int64_t i = c0 + threadIdx.x;
-        const uint64_t v = (i < a.nb) ? a.block_tot[i] : 0;
-        const uint64_t ex = wave_excl_scan_int<uint64_t>(v);
-        const uint64_t wt = wave_sum_int<uint64_t>(v);
-        __shared__ uint64_t wsum[16];
-        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = wt;
-        __syncthreads();
-        uint64_t wbase = 0;
-        for (int k = 0; k < (int)(threadIdx.x >> 6); ++k) wbase += wsum[k];
-        const uint64_t carry = carry_sh;
-        if (i < a.nb) a.block_off[i] = carry + wbase + ex;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint64_t t = 0;
-            for (int k = 0; k < 16; ++k) t += wsum[k];
-            carry_sh = carry + t;
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const uint64_t total = carry_sh;
-        a.st->rej_size = total;
-        if (total > a.A_cap) atomicOr(&a.st->error, ERR_BAD_SUM);
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
-    __shared__ uint64_t incl[REJ_ITEMS];
-    __shared__ uint64_t wsum[REJ_THREADS / 64];
-    __shared__ int32_t big[REJ_ITEMS];
-    __shared__ uint32_t nbig;
-    if (a.st->error) return;
-    const double W = a.st->W;
-    const T* w = (const T*)a.weights;
-    const int64_t base = (int64_t)blockIdx.x * REJ_ITEMS;
-    const int cnt = (int)min<int64_t>(REJ_ITEMS, a.n - base);
-    if (threadIdx.x == 0) nbig = 0;
-    // Per-thread blocked items (16 consecutive), inclusive offsets in LDS.
-    const int per = REJ_ITEMS / REJ_THREADS;
-    uint64_t loc[REJ_ITEMS / REJ_THREADS];
+    __shared__ uint64_t wsum[16];
+    const int64_t q = (a.nb + 1023) / 1024;
+    const int64_t b0 = min<int64_t>(a.nb, (int64_t)threadIdx.x * q), b1 = min<int64_t>(a.nb, b0 + q);
     uint64_t run = 0;
-    for (int u = 0; u < per; ++u) {
-        const int k = threadIdx.x * per + u;
-        run += (k < cnt) ? rej_count<T>(a, base + k, W) : 0u;
-        loc[u] = run;
-    }
+    for (int64_t i = b0; i < b1; ++i) run += a.block_tot[i];
     const uint64_t ex = wave_excl_scan_int<uint64_t>(run);
     const uint64_t wt = wave_sum_int<uint64_t>(run);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = wt;
     __syncthreads();
-    uint64_t wbase = 0;
-    for (int k = 0; k < (int)(threadIdx.x >> 6); ++k) wbase += wsum[k];
-    const uint64_t boff = a.block_off[blockIdx.x];
-    for (int u = 0; u < per; ++u) incl[threadIdx.x * per + u] = boff + wbase + ex + loc[u];
-    if (a.incl_out)
-        for (int u = 0; u < per; ++u) {
-            const int k = threadIdx.x * per + u;
-            if (k < cnt) a.incl_out[base + k] = incl[k];
-        }
+    uint64_t off = ex;
+    for (int k = 0; k < (int)(threadIdx.x >> 6); ++k) off += wsum[k];
+    for (int64_t i = b0; i < b1; ++i) {
+        a.block_off[i] = off;
+        off += a.block_tot[i];
+    }
+    if (threadIdx.x == 1023) {
+        // The last thread's running offset is the total: its run ends at nb
+        // (or is empty, past nb, after every non-empty run).
+        a.st->rej_size = off;
+        a.st->rej_tasks = 0;
+        if (off > a.A_cap) atomicOr(&a.st->error, ERR_BAD_SUM);
+    }
+}
+
+// A for the block's REJ_ITEMS items.  Wave v owns items [1024 v, 1024 v +
+// 1024) of the block and reads them in 16 rounds of 64 (coalesced); the
+// counts stay in registers.  One exchange of the waves' totals gives each
+// wave its base; then every wave scans its rounds (integers: exact in any
+// order) and writes each item's run [first, first + c).  The first slot
+// carries the item's acceptance fraction (FP64; the reference computes it in
+// float, RejectionSampling.cu:26-28), the others accept always.  Runs of up to
+// REJ_SHORT slots are written by their lane, up to REJ_WAVE by the wave; the
+// rest of a longer run becomes tasks of REJ_TASK slots for rej_fill_long, so
+// that heavy items, however clustered, hold up no wave for long.
+template <typename T>
+__global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
+    constexpr int U = REJ_ITEMS / REJ_THREADS;
+    __shared__ uint64_t wtot[REJ_THREADS / 64];
+    if (a.st->error) return;
+    const double W = a.st->W;
+    const T* w = (const T*)a.weights;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * REJ_ITEMS + (int64_t)wave * 64 * U;
+    const int cnt = (int)max<int64_t>(0, min<int64_t>(64 * U, a.n - base));
+    T wr[U];  // raw values: a conversion next to each load would serialise the loads
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int k = u * 64 + lane;
+        wr[u] = (k < cnt) ? w[base + k] : T(0);
+    }
+    uint32_t c[U];
+    uint64_t total = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const double x = ((double)a.n * (double)wr[u]) / W;
+        c[u] = (u * 64 + lane < cnt) ? (uint32_t)floor(x) + 1u : 0u;
+        total += c[u];
+    }
+    total = wave_sum_int<uint64_t>(total);
+    if (lane == 0) wtot[wave] = total;
     __syncthreads();
-    // Each item's run [incl[k-1], incl[k]) of A.  The first slot carries the
-    // item's acceptance fraction (FP64; the reference computes it in float,
-    // RejectionSampling.cu:26-28), the others accept always.  Runs of up to
-    // REJ_SHORT slots are written by one thread, items striped over the
-    // threads so that a wave's stores are nearly contiguous; longer runs are
-    // listed and written by the whole block.
-    for (int k = threadIdx.x; k < cnt; k += REJ_THREADS) {
-        const uint64_t first = k == 0 ? boff : incl[k - 1];
-        const uint64_t len = incl[k] - first;
-        const double x = ((double)a.n * (double)w[base + k]) / W;
-        RejEntry e;
-        e.accept = x - floor(x);
-        e.item = (int32_t)(base + k);
-        e.pad = 0;
-        *reinterpret_cast<uint4*>(&a.A[first]) = *reinterpret_cast<const uint4*>(&e);
-        if (len > REJ_SHORT) {
-            big[atomicAdd(&nbig, 1u)] = k;
-        } else {
+    uint64_t first0 = a.block_off[blockIdx.x];
+    for (int q = 0; q < wave; ++q) first0 += wtot[q];
+    for (int u = 0; u < U; ++u) {
+        const int k = u * 64 + lane;
+        const uint64_t cu = c[u];
+        const uint64_t first = first0 + wave_excl_scan_int<uint64_t>(cu);
+        first0 += wave_sum_int<uint64_t>(cu);
+        const int32_t item = (int32_t)(base + k);
+        if (k < cnt) {
+            const double x = ((double)a.n * (double)wr[u]) / W;
+            if (a.incl_out) a.incl_out[base + k] = first + cu;
+            RejEntry e;
+            e.accept = x - floor(x);
+            e.item = item;
+            e.pad = 0;
+            *reinterpret_cast<uint4*>(&a.A[first]) = *reinterpret_cast<const uint4*>(&e);
+            if (cu <= REJ_SHORT) {
+                e.accept = 1.0;
+                for (uint64_t q = 1; q < cu; ++q) *reinterpret_cast<uint4*>(&a.A[first + q]) = *reinterpret_cast<const uint4*>(&e);
+            }
+        }
+        const bool wave_run = k < cnt && cu > REJ_SHORT && cu <= REJ_WAVE;
+        uint64_t m = __ballot(wave_run);
+        while (m != 0ull) {
+            const int b = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const uint64_t f = readlane_u64(first, b), len = readlane_u64(cu, b);
+            RejEntry e;
             e.accept = 1.0;
-            for (uint64_t q = 1; q < len; ++q) *reinterpret_cast<uint4*>(&a.A[first + q]) = *reinterpret_cast<const uint4*>(&e);
+            e.item = (int32_t)(base + u * 64 + b);
+            e.pad = 0;
+            if (1 + lane < len) *reinterpret_cast<uint4*>(&a.A[f + 1 + lane]) = *reinterpret_cast<const uint4*>(&e);
+        }
+        const bool long_run = k < cnt && cu > REJ_WAVE;
+        if (__ballot(long_run) != 0ull) {
+            const uint32_t nt = long_run ? (uint32_t)((cu - 1 + REJ_TASK - 1) / REJ_TASK) : 0u;
+            const uint32_t tex = (uint32_t)wave_excl_scan_int<uint64_t>(nt);
+            const uint32_t tsum = (uint32_t)wave_sum_int<uint64_t>(nt);
+            uint32_t t0 = 0;
+            if (lane == 0) t0 = atomicAdd(&a.st->rej_tasks, tsum);
+            t0 = __shfl(t0, 0) + tex;
+            if (long_run) {
+                if ((uint64_t)t0 + nt > a.tasks_cap) {
+                    atomicOr(&a.st->error, ERR_BAD_SUM);
+                } else {
+                    for (uint32_t j = 0; j < nt; ++j) {
+                        RejTask t;
+                        t.first = first + 1 + (uint64_t)j * REJ_TASK;
+                        t.len = (uint32_t)min<uint64_t>(REJ_TASK, cu - 1 - (uint64_t)j * REJ_TASK);
+                        t.item = item;
+                        *reinterpret_cast<uint4*>(&a.tasks[t0 + j]) = *reinterpret_cast<const uint4*>(&t);
+                    }
+                }
+            }
         }
     }
-    __syncthreads();
-    for (uint32_t b = 0; b < nbig; ++b) {
-        const int k = big[b];
-        const uint64_t first = k == 0 ? boff : incl[k - 1];
-        const uint64_t len = incl[k] - first;
+}
+
+// The tasks of rej_fill: one wave per task, REJ_TASK slots with acceptance 1.0.
+__global__ __launch_bounds__(256) void rej_fill_long(RejArgs a) {
+    if (a.st->error) return;
+    const uint32_t nt = min(a.st->rej_tasks, a.tasks_cap);
+    const int lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t t = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); t < nt; t += waves) {
+        const RejTask k = a.tasks[t];
         RejEntry e;
         e.accept = 1.0;
-        e.item = (int32_t)(base + k);
+        e.item = k.item;
         e.pad = 0;
-        for (uint64_t q = 1 + threadIdx.x; q < len; q += REJ_THREADS)
-            *reinterpret_cast<uint4*>(&a.A[first + q]) = *reinterpret_cast<const uint4*>(&e);
+        for (uint32_t q = lane; q < k.len; q += 64)
+            *reinterpret_cast<uint4*>(&a.A[k.first + q]) = *reinterpret_cast<const uint4*>(&e);
     }
 }
 
@@ -349,15 +405,22 @@ hipError_t launch_rejection_build(const RejArgs& a, hipStream_t s) {
         hipLaunchKernelGGL(rej_block_totals<float>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
         hipLaunchKernelGGL(rej_scan_totals, dim3(1), dim3(1024), 0, s, a);
         hipLaunchKernelGGL(rej_fill<float>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
+        hipLaunchKernelGGL(rej_fill_long, dim3(REJ_LONG_BLOCKS), dim3(256), 0, s, a);
     } else {
         hipLaunchKernelGGL(rej_block_totals<double>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
         hipLaunchKernelGGL(rej_scan_totals, dim3(1), dim3(1024), 0, s, a);
         hipLaunchKernelGGL(rej_fill<double>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
+        hipLaunchKernelGGL(rej_fill_long, dim3(REJ_LONG_BLOCKS), dim3(256), 0, s, a);
     }
     return hipGetLastError();
 }
 
 int64_t rejection_blocks(int64_t n) { return (n + REJ_ITEMS - 1) / REJ_ITEMS; }
+
+// An item with a run of len > REJ_WAVE slots makes ceil((len - 1) / REJ_TASK)
+// <= (len - 1) / REJ_TASK + (len - 1) / REJ_WAVE tasks, so |A| <= A_cap slots
+// make at most A_cap (1 / REJ_TASK + 1 / REJ_WAVE) tasks.
+uint32_t rejection_task_cap(uint64_t A_cap) { return (uint32_t)(A_cap / REJ_TASK + A_cap / REJ_WAVE + 1); }
 
 hipError_t launch_rejection_sample(const RejArgs& a, const SampleArgs& sa, hipStream_t s) {
     const unsigned g = grid_for(sa.count, 256, 256 * 64);

@@ -32,7 +32,7 @@ struct DevState {
     uint64_t split_fallbacks;
     uint64_t learned_fallbacks;
     uint64_t rej_size;        // |A| of the rejection structure
-    uint32_t pad3;
+    uint32_t rej_tasks;       // long-run tasks of the rejection fill
     uint32_t pad1;
 };
 
@@ -46,6 +46,14 @@ struct alignas(16) Row {
 // One slot of the rejection array A (RejectionSampling.cu:26-56): the item and
 // the acceptance threshold of the slot: frac(N w / W) for the first slot of
 // the item's run, 1.0 (always accept) for the others.
+// A run of A slots [first, first + len) that all hold `item` with acceptance
+// 1.0: part of a long run, written by the whole grid.
+struct alignas(16) RejTask {
+    uint64_t first;
+    uint32_t len;
+    int32_t item;
+};
+
 struct alignas(16) RejEntry {
     double accept;
     int32_t item;

@@ -77,7 +77,9 @@ def test_histograms_chi_squared(gpu, oracle):
     assert chi2_pvalue(h, w) > 1e-4
 
 
-@pytest.mark.parametrize("n,dist", [(1000, "sine"), (100003, "pareto"), (1 << 20, "exponential")])
+@pytest.mark.parametrize("n,dist", [(1000, "sine"), (100003, "pareto"), (1 << 20, "exponential"),
+                                    (50_000, "one_heavy"), (300_001, "heavy_mix"), (1, "unit_uniform"),
+                                    (4097, "powerlaw2")])
 def test_rejection_matches_oracle(gpu, oracle, n, dist):
     torch = gpu
     w = atg.generate_weights(dist, n, seed=11, param=1.1 if dist == "pareto" else 0.0)
