@@ -111,6 +111,18 @@ __device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgc
 // returned to every lane.
 __device__ __forceinline__ double wave_tree_sum(double v) { return readlane_d(wave_scan(v), 63); }
 
+// Inclusive scan of 32-bit integers over the wave, with the DPP steps of
+// wave_scan (integer sums are exact in any order).
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum_int(T v) {
 #pragma unroll

@@ -153,8 +153,7 @@ __global__ __launch_bounds__(256) void sample_sectioned(SampleArgs a) {
 // slots of long runs are cut into tasks that the whole grid writes.
 #define REJ_ITEMS 4096
 #define REJ_THREADS 256
-#define REJ_SHORT 8     // runs up to this are written by their lane
-#define REJ_WAVE 64     // runs up to this by their wave, longer ones as tasks
+#define REJ_WAVE 64     // runs up to this are written by their wave, longer ones as tasks
 #define REJ_TASK 4096   // slots per task
 #define REJ_LONG_BLOCKS 1024
 
@@ -231,10 +230,11 @@ __global__ __launch_bounds__(1024) void rej_scan_totals(RejArgs a) {
 // wave its base; then every wave scans its rounds (integers: exact in any
 // order) and writes each item's run [first, first + c).  The first slot
 // carries the item's acceptance fraction (FP64; the reference computes it in
-// float, RejectionSampling.cu:26-28), the others accept always.  Runs of up to
-// REJ_SHORT slots are written by their lane, up to REJ_WAVE by the wave; the
-// rest of a longer run becomes tasks of REJ_TASK slots for rej_fill_long, so
-// that heavy items, however clustered, hold up no wave for long.
+// float, RejectionSampling.cu:26-28), the others accept always.  The wave
+// writes the runs of a round together, consecutive slots by consecutive
+// lanes; the rest of a run longer than REJ_WAVE slots becomes tasks of
+// REJ_TASK slots for rej_fill_long, so that heavy items, however clustered,
+// hold up no wave for long.
 template <typename T>
 __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
     constexpr int U = REJ_ITEMS / REJ_THREADS;
@@ -266,40 +266,54 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
     for (int q = 0; q < wave; ++q) first0 += wtot[q];
     for (int u = 0; u < U; ++u) {
         const int k = u * 64 + lane;
-        const uint64_t cu = c[u];
-        const uint64_t first = first0 + wave_excl_scan_int<uint64_t>(cu);
-        first0 += wave_sum_int<uint64_t>(cu);
+        // A round's counts sum to at most |A| <= 2n < 2^32: 32-bit scans.
+        const uint32_t cu = c[u];
+        const uint32_t incl = wave_incl_scan_u32(cu);
+        const uint64_t first = first0 + (incl - cu);
+        first0 += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         const int32_t item = (int32_t)(base + k);
-        if (k < cnt) {
-            const double x = ((double)a.n * (double)wr[u]) / W;
-            if (a.incl_out) a.incl_out[base + k] = first + cu;
-            RejEntry e;
-            e.accept = x - floor(x);
-            e.item = item;
-            e.pad = 0;
-            *reinterpret_cast<uint4*>(&a.A[first]) = *reinterpret_cast<const uint4*>(&e);
-            if (cu <= REJ_SHORT) {
-                e.accept = 1.0;
-                for (uint64_t q = 1; q < cu; ++q) *reinterpret_cast<uint4*>(&a.A[first + q]) = *reinterpret_cast<const uint4*>(&e);
+        const double x = ((double)a.n * (double)wr[u]) / W;
+        const double frac = x - floor(x);
+        if (k < cnt && a.incl_out) a.incl_out[base + k] = first + cu;
+        // The round's slots in the wave, long runs counted by their first slot
+        // only: lane v of each pass writes the v-th of them, owned by the
+        // last item whose local start is <= v (a search over the 64 starts).
+        const bool long_run = k < cnt && cu > REJ_WAVE;
+        const uint64_t any_long = __ballot(long_run);
+        uint32_t lex = incl - cu, L = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        if (any_long != 0ull) {
+            const uint32_t ell = long_run ? 1u : cu;
+            const uint32_t il = wave_incl_scan_u32(ell);
+            lex = il - ell;
+            L = (uint32_t)__builtin_amdgcn_readlane((int)il, 63);
+        }
+        for (uint32_t v0 = 0; v0 < L; v0 += 64) {
+            const uint32_t v = v0 + lane;
+            int o = 0;
+            uint32_t lo_start = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1) {
+                const uint32_t st = (uint32_t)__shfl((int)lex, o + step, 64);
+                if (st <= v) {
+                    o += step;
+                    lo_start = st;
+                }
+            }
+            const uint64_t f = (uint64_t)__shfl((unsigned long long)first, o, 64);
+            const double fr = __shfl(frac, o, 64);
+            if (v < L) {
+                RejEntry e;
+                e.accept = (v == lo_start) ? fr : 1.0;
+                e.item = (int32_t)(base + u * 64 + o);
+                e.pad = 0;
+                *reinterpret_cast<uint4*>(&a.A[f + (v - lo_start)]) = *reinterpret_cast<const uint4*>(&e);
             }
         }
-        const bool wave_run = k < cnt && cu > REJ_SHORT && cu <= REJ_WAVE;
-        uint64_t m = __ballot(wave_run);
-        while (m != 0ull) {
-            const int b = __ffsll((long long)m) - 1;
-            m &= m - 1;
-            const uint64_t f = readlane_u64(first, b), len = readlane_u64(cu, b);
-            RejEntry e;
-            e.accept = 1.0;
-            e.item = (int32_t)(base + u * 64 + b);
-            e.pad = 0;
-            if (1 + lane < len) *reinterpret_cast<uint4*>(&a.A[f + 1 + lane]) = *reinterpret_cast<const uint4*>(&e);
-        }
-        const bool long_run = k < cnt && cu > REJ_WAVE;
-        if (__ballot(long_run) != 0ull) {
-            const uint32_t nt = long_run ? (uint32_t)((cu - 1 + REJ_TASK - 1) / REJ_TASK) : 0u;
-            const uint32_t tex = (uint32_t)wave_excl_scan_int<uint64_t>(nt);
-            const uint32_t tsum = (uint32_t)wave_sum_int<uint64_t>(nt);
+        if (any_long != 0ull) {
+            const uint32_t nt = long_run ? (cu - 1 + REJ_TASK - 1) / REJ_TASK : 0u;
+            const uint32_t ti = wave_incl_scan_u32(nt);
+            const uint32_t tex = ti - nt;
+            const uint32_t tsum = (uint32_t)__builtin_amdgcn_readlane((int)ti, 63);
             uint32_t t0 = 0;
             if (lane == 0) t0 = atomicAdd(&a.st->rej_tasks, tsum);
             t0 = __shfl(t0, 0) + tex;
@@ -310,7 +324,7 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
                     for (uint32_t j = 0; j < nt; ++j) {
                         RejTask t;
                         t.first = first + 1 + (uint64_t)j * REJ_TASK;
-                        t.len = (uint32_t)min<uint64_t>(REJ_TASK, cu - 1 - (uint64_t)j * REJ_TASK);
+                        t.len = min<uint32_t>(REJ_TASK, cu - 1 - j * REJ_TASK);
                         t.item = item;
                         *reinterpret_cast<uint4*>(&a.tasks[t0 + j]) = *reinterpret_cast<const uint4*>(&t);
                     }
