@@ -157,6 +157,13 @@ __global__ __launch_bounds__(256) void sample_sectioned(SampleArgs a) {
 #define REJ_TASK 4096   // slots per task
 #define REJ_LONG_BLOCKS 1024
 
+// A is written once and read only by a later sampling launch: streaming
+// (non-temporal) 16-byte stores.
+typedef unsigned int rej_u4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void rej_store(RejEntry* p, const RejEntry& e) {
+    __builtin_nontemporal_store(*reinterpret_cast<const rej_u4*>(&e), reinterpret_cast<rej_u4*>(p));
+}
+
 template <typename T>
 __device__ __forceinline__ uint32_t rej_count(const RejArgs& a, int64_t i, double W) {
     const double x = ((double)a.n * (double)((const T*)a.weights)[i]) / W;
@@ -176,7 +183,7 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_block_totals(RejArgs a) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int k = u * REJ_THREADS + threadIdx.x;
-        wr[u] = (k < cnt) ? w[base + k] : T(0);
+        wr[u] = (k < cnt) ? __builtin_nontemporal_load(&w[base + k]) : T(0);
     }
     uint64_t acc = 0;
 #pragma unroll
@@ -203,17 +210,31 @@ __global__ __launch_bounds__(1024) void rej_scan_totals(RejArgs a) {
     __shared__ uint64_t wsum[16];
     const int64_t q = (a.nb + 1023) / 1024;
     const int64_t b0 = min<int64_t>(a.nb, (int64_t)threadIdx.x * q), b1 = min<int64_t>(a.nb, b0 + q);
+    // Batches of 16 loads in flight: the run's loads would otherwise wait one
+    // after another.
     uint64_t run = 0;
-    for (int64_t i = b0; i < b1; ++i) run += a.block_tot[i];
+    for (int64_t i0 = b0; i0 < b1; i0 += 16) {
+        uint64_t v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = (i0 + j < b1) ? a.block_tot[i0 + j] : 0ull;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) run += v[j];
+    }
     const uint64_t ex = wave_excl_scan_int<uint64_t>(run);
     const uint64_t wt = wave_sum_int<uint64_t>(run);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = wt;
     __syncthreads();
     uint64_t off = ex;
     for (int k = 0; k < (int)(threadIdx.x >> 6); ++k) off += wsum[k];
-    for (int64_t i = b0; i < b1; ++i) {
-        a.block_off[i] = off;
-        off += a.block_tot[i];
+    for (int64_t i0 = b0; i0 < b1; i0 += 16) {
+        uint64_t v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = (i0 + j < b1) ? a.block_tot[i0 + j] : 0ull;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (i0 + j < b1) a.block_off[i0 + j] = off;
+            off += v[j];
+        }
     }
     if (threadIdx.x == 1023) {
         // The last thread's running offset is the total: its run ends at nb
@@ -249,7 +270,7 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int k = u * 64 + lane;
-        wr[u] = (k < cnt) ? w[base + k] : T(0);
+        wr[u] = (k < cnt) ? __builtin_nontemporal_load(&w[base + k]) : T(0);
     }
     uint32_t c[U];
     uint64_t total = 0;
@@ -306,7 +327,7 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
                 e.accept = (v == lo_start) ? fr : 1.0;
                 e.item = (int32_t)(base + u * 64 + o);
                 e.pad = 0;
-                *reinterpret_cast<uint4*>(&a.A[f + (v - lo_start)]) = *reinterpret_cast<const uint4*>(&e);
+                rej_store(&a.A[f + (v - lo_start)], e);
             }
         }
         if (any_long != 0ull) {
@@ -347,7 +368,7 @@ __global__ __launch_bounds__(256) void rej_fill_long(RejArgs a) {
         e.item = k.item;
         e.pad = 0;
         for (uint32_t q = lane; q < k.len; q += 64)
-            *reinterpret_cast<uint4*>(&a.A[k.first + q]) = *reinterpret_cast<const uint4*>(&e);
+            rej_store(&a.A[k.first + q], e);
     }
 }
 
