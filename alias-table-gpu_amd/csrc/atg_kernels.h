@@ -405,8 +405,11 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
                 if (r < m) {
                     v[u] = heavy ? L.Hw(pos[u]) : L.Lw(pos[u]);
                     const int it = heavy ? L.Hi(pos[u]) : L.Li(pos[u]);
-                    a.stage_w[start + rank0 + r] = v[u];
-                    a.stage_i[start + rank0 + r] = (int32_t)(start + it);
+                    // Streaming stores: the staging is read once, by the
+                    // placement pass (non-temporal row or list stores were
+                    // slower: those lines are merged or re-read in L2).
+                    __builtin_nontemporal_store(v[u], &a.stage_w[start + rank0 + r]);
+                    __builtin_nontemporal_store((int32_t)(start + it), &a.stage_i[start + rank0 + r]);
                 }
             }
 #pragma unroll
@@ -461,8 +464,8 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
                 v[u] = 0.0;
                 item[u] = 0;
                 if (r < m) {
-                    v[u] = a.stage_w[src + r];
-                    item[u] = a.stage_i[src + r];
+                    v[u] = __builtin_nontemporal_load(&a.stage_w[src + r]);
+                    item[u] = __builtin_nontemporal_load(&a.stage_i[src + r]);
                 }
             }
 #pragma unroll
