@@ -58,8 +58,9 @@ __device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t
 // placing them.
 #define PLACE_GROUP 8
 // Chunks of 64 leftovers the placement pass keeps in flight (a 2048-item
-// section has at most 32).
-#define PLACE_CHUNKS 16
+// section has at most 32).  8 rather than 16: fewer registers, more waves,
+// the same loads in flight per SIMD (1% faster placement).
+#define PLACE_CHUNKS 8
 
 struct SecLDS {
     double* w;      // [Ls]: lights from the front, heavies from the back

@@ -28,8 +28,8 @@ HBM_PEAK_GBS = 8000.0
 # (calibrated on a 1 GiB device copy) for the headline workload at this shape;
 # profiles/build_pareto1e8_s2048_c24_p1562500_hbm_traffic.md has the commands
 # and counts.
-MEASURED_TRAFFIC = {(100_000_000, 2048, 24, 1562500): {"sum": 803_000_000, "partition": 4_232_000_000,
-                                                       "split": 3_004_000_000, "pack": 1_822_000_000}}
+MEASURED_TRAFFIC = {(100_000_000, 2048, 24, 1562500): {"sum": 804_000_000, "partition": 4_249_000_000,
+                                                       "split": 3_007_000_000, "pack": 1_821_000_000}}
 
 
 def log(*a):
