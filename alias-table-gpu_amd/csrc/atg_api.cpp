@@ -83,6 +83,9 @@ struct atg_table_s {
     uint64_t A_cap = 0;
     uint64_t *d_rej_tot = nullptr, *d_rej_off = nullptr;
     RejTask* d_rej_tasks = nullptr;
+    // sectioned sampling: count and first output of each section
+    uint64_t* d_sec_counts = nullptr;
+    int64_t sec_counts_cap = 0;
     bool rej_built = false;
     // host side
     std::vector<double> h_weights;  // host copy when weights were given on the host
@@ -156,7 +159,7 @@ static void free_table(atg_table* t) {
     void* ptrs[] = {t->d_weights_owned, t->d_state, t->d_section_sums, t->d_tree, t->d_slots, t->d_sec_cert, t->d_stage_w,
                     t->d_stage_i, t->d_Li, t->d_Hi,
                     t->d_Lw, t->d_Lp, t->d_Hw, t->d_Hp, t->d_rows, t->d_tw, t->d_ta, t->d_splits, t->d_knots,
-                    t->d_checksum, t->d_A, t->d_rej_tot, t->d_rej_off, t->d_rej_tasks};
+                    t->d_checksum, t->d_A, t->d_rej_tot, t->d_rej_off, t->d_rej_tasks, t->d_sec_counts};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->events)
@@ -617,6 +620,19 @@ atg_status atg_sample(atg_table* t, atg_sample_kind kind, uint64_t count, uint64
             if (section_rows > 8192) return fail(ATG_EINVAL, "atg_sample: section_rows %lld > 8192", (long long)section_rows);
             a.SS = section_rows;
             a.nsec = (t->n + section_rows - 1) / section_rows;
+            if (a.nsec > t->sec_counts_cap) {
+                // Grows only; freed with the table.
+                HIPCHK(hipStreamSynchronize(s));
+                if (t->d_sec_counts) HIPCHK(hipFree(t->d_sec_counts));
+                t->d_sec_counts = nullptr;
+                t->sec_counts_cap = 0;
+                if (dalloc(&t->d_sec_counts, (size_t)(2 * a.nsec)) != hipSuccess) {
+                    (void)hipGetLastError();
+                    return fail(ATG_ENOMEM, "atg_sample: section counts");
+                }
+                t->sec_counts_cap = a.nsec;
+            }
+            a.sec_counts = t->d_sec_counts;
             HIPCHK(set_sample_lds_limit((int)(section_rows * sizeof(Row))));
             HIPCHK(launch_sample_sectioned(a, s));
         } else {

@@ -31,6 +31,7 @@ struct SampleArgs {
     unsigned long long* checksum;  // nullable (device)
     int64_t SS;            // sectioned: rows per section
     int64_t nsec;
+    uint64_t* sec_counts;  // sectioned: 2 nsec values, count and first output of each section
 };
 
 struct RejArgs {

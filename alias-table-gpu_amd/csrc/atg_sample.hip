@@ -109,22 +109,27 @@ __device__ void section_count(const SampleArgs& a, int64_t s, uint64_t& cnt, uin
 }
 
 // A section of up to 8192 rows (64 KB at the default 4096) fills a large part
-// of a CU's LDS, so few blocks are resident: 512 threads per block keep
-// enough waves per SIMD to hide the latency of the Philox chains (256 threads
-// were 17% slower, 1024 5%, at 1e9 samples from 1e8 rows).
-#define SECTIONED_THREADS 512
+// of a CU's LDS, so few blocks are resident: 1024 threads per block keep
+// enough waves per SIMD to hide the latency of the Philox chains (512 threads
+// were 7% slower, 256 32%, at 1e9 samples from 1e8 rows).
+#define SECTIONED_THREADS 1024
+
+// Count and first output of every section, one thread per section: each
+// descent is a serial chain of binomial draws, which would otherwise hold up
+// a whole sampling block at its start.
+__global__ __launch_bounds__(64) void section_counts(SampleArgs a) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nsec) return;
+    uint64_t c, o;
+    section_count(a, s, c, o);
+    a.sec_counts[2 * s] = c;
+    a.sec_counts[2 * s + 1] = o;
+}
 
 template <bool AOS>
 __global__ __launch_bounds__(SECTIONED_THREADS) void sample_sectioned(SampleArgs a) {
     extern __shared__ Row sect[];
-    __shared__ uint64_t sh_cnt, sh_off;
     const int64_t s = blockIdx.x;
-    if (threadIdx.x == 0) {
-        uint64_t c, o;
-        section_count(a, s, c, o);
-        sh_cnt = c;
-        sh_off = o;
-    }
     const int64_t start = s * a.SS;
     const int64_t len = min(a.n, start + a.SS) - start;
     // Stage the section's rows in LDS (SamplerGpuSectionedShared.cu:58-63).
@@ -138,7 +143,7 @@ __global__ __launch_bounds__(SECTIONED_THREADS) void sample_sectioned(SampleArgs
         }
     }
     __syncthreads();
-    const uint64_t cnt = sh_cnt, off = sh_off;
+    const uint64_t cnt = a.sec_counts[2 * s], off = a.sec_counts[2 * s + 1];
     const double W_N = a.st->W_N;
     uint64_t csum = 0;
     for (uint64_t t = threadIdx.x; t < cnt; t += blockDim.x) {
@@ -427,6 +432,7 @@ hipError_t launch_sample_basic(const SampleArgs& a, hipStream_t s) {
 
 hipError_t launch_sample_sectioned(const SampleArgs& a, hipStream_t s) {
     const size_t lds = (size_t)a.SS * sizeof(Row);
+    hipLaunchKernelGGL(section_counts, dim3((unsigned)((a.nsec + 63) / 64)), dim3(64), 0, s, a);
     if (a.rows) hipLaunchKernelGGL(sample_sectioned<true>, dim3((unsigned)a.nsec), dim3(SECTIONED_THREADS), lds, s, a);
     else hipLaunchKernelGGL(sample_sectioned<false>, dim3((unsigned)a.nsec), dim3(SECTIONED_THREADS), lds, s, a);
     return hipGetLastError();
