@@ -56,6 +56,20 @@ def test_sectioned_matches_oracle(gpu, oracle, n, count, rows):
     np.testing.assert_array_equal(got, exp)
 
 
+def test_sectioned_section_sizes_on_one_table(gpu, oracle):
+    """Calls with more sections than any before on the same table (the
+    per-section counts buffer grows), then fewer again."""
+    torch = gpu
+    n, count = 100003, 500000
+    w, t, ref = build_pair(oracle, n)
+    out = torch.empty(count, dtype=torch.int32, device="cuda")
+    for rows, seed in ((4096, 7), (64, 8), (4096, 9), (1, 10)):
+        t.sample(count, seed=seed, kind="sectioned", section_rows=rows, out=out)
+        got = out.cpu().numpy()
+        exp = oracle.sample_sectioned(ref["row_w"], ref["row_alias"], ref["stats"]["W_N"], rows, count, seed)
+        np.testing.assert_array_equal(got, exp, err_msg=f"section_rows={rows}")
+
+
 def test_histograms_chi_squared(gpu, oracle):
     torch = gpu
     n = 10000
