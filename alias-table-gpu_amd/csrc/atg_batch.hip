@@ -51,10 +51,11 @@ struct BatchSampleArgs {
     int32_t* out;             // ntables * per_table
 };
 
-// Blocks of the degenerate-table fill per table (a grid-stride loop covers
-// the rest; most tables are not degenerate and leave at once).
+// Blocks of the degenerate-table fill per table: one per 65,536 items, at
+// most 8 (a grid-stride loop covers the rest).  Most tables are not
+// degenerate and leave at once, so few blocks keep the launch short.
 __host__ __device__ inline int64_t degenerate_blocks(int64_t n) {
-    const int64_t b = (n + 2 + 255) / 256;
+    const int64_t b = (n + 2 + 65535) / 65536;
     return b < 8 ? b : 8;
 }
 
@@ -138,9 +139,8 @@ __global__ __launch_bounds__(256) void seg_place(SegArgs g, SegMap m) {
 __global__ __launch_bounds__(256) void seg_degenerate(SegArgs g, SegMap m) {
     int64_t t, blk;
     seg_locate(m, t, blk);
-    const BuildArgs& a = g.tables[t];
-    tick_end(g, t);
-    degenerate_block<true>(a, blk, degenerate_blocks(a.n));
+    // No latency tick here: every table has pack blocks, which run later.
+    degenerate_block<true>(g.tables[t], blk, degenerate_blocks(g.tables[t].n));
 }
 
 __global__ __launch_bounds__(256) void seg_split(SegArgs g, SegMap m) {
