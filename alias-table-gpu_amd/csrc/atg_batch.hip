@@ -438,7 +438,7 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
             case MAP_SUM: return (z.G + 3) / 4;
             case MAP_LOCAL: return z.G;
             case MAP_PLACE: return (z.G + 3) / 4;
-            case MAP_TREE: return (z.G + 511) / 512;
+            case MAP_TREE: return z.G > 1 ? (z.G + 511) / 512 : 0;  // one section: no tree
             case MAP_DEG: return degenerate_blocks(n);
             case MAP_SPLIT: return std::max<int64_t>(1, (z.p_req - 1 + 255) / 256);
             case MAP_KNOTS: return (K + 1 + 255) / 256;
