@@ -109,3 +109,24 @@ def test_rejection_matches_oracle(gpu, oracle, n, dist):
     t.rejection_sample(count, seed=8, out=out)
     exp, _ = oracle.sample_rejection(A, w, W, 8, count)
     np.testing.assert_array_equal(out.cpu().numpy(), exp)
+
+
+@pytest.mark.parametrize("n,dist", [(100003, "pareto"), (70001, "one_heavy")])
+def test_rejection_f32_weights(gpu, oracle, n, dist):
+    """FP32 weights: the counts and acceptance fractions use the exact FP64
+    values of the FP32 inputs."""
+    torch = gpu
+    w = atg.generate_weights(dist, n, seed=13, param=1.1 if dist == "pareto" else 0.0).astype(np.float32)
+    w64 = w.astype(np.float64)
+    t = atg.Table(n, dtype="f32")
+    t.set_weights(w)
+    t.rejection_build()
+    W = oracle.weight_sum(w64, max(1, min(1000, n // 10)))[0]
+    incl, A = oracle.rejection_build(w64, W)
+    assert t.rejection_size() == len(A)
+    np.testing.assert_array_equal(t.rejection_array(), A)
+    count = 500_000
+    out = torch.empty(count, dtype=torch.int32, device="cuda")
+    t.rejection_sample(count, seed=21, out=out)
+    exp, _ = oracle.sample_rejection(A, w64, W, 21, count)
+    np.testing.assert_array_equal(out.cpu().numpy(), exp)
