@@ -244,7 +244,7 @@ struct SectionOut {
     int64_t nL, nH;  /* leftover counts, stored at temp[start, start+n) */
 };
 
-static SectionOut process_section(const double* w, int64_t n, int64_t start, int64_t len, double W_N,
+static SectionOut process_section(const double* w, int64_t start, int64_t len, double W_N,
                                   int32_t psa_plus, int32_t C, double* row_w, int32_t* row_alias,
                                   int32_t* tLi, double* tLw, int32_t* tHi, double* tHw) {
     std::vector<int32_t> li, hi;
@@ -364,7 +364,7 @@ extern "C" int oracle_build(const double* w, int64_t n, const oracle_opts* o, do
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t s = 0; s < G; ++s) {
         const int64_t a = std::min(n, s * Ls), b = std::min(n, (s + 1) * Ls);
-        SectionOut so = process_section(w, n, a, b - a, W_N, o->psa_plus, o->chains, row_w, row_alias,
+        SectionOut so = process_section(w, a, b - a, W_N, o->psa_plus, o->chains, row_w, row_alias,
                                         tLi.data(), tLw.data(), tHi.data(), tHw.data());
         cntL[s] = so.nL; cntH[s] = so.nH;
     }
