@@ -175,12 +175,6 @@ __device__ __forceinline__ void rej_store(RejEntry* p, const RejEntry& e) {
     __builtin_nontemporal_store(*reinterpret_cast<const rej_u4*>(&e), reinterpret_cast<rej_u4*>(p));
 }
 
-template <typename T>
-__device__ __forceinline__ uint32_t rej_count(const RejArgs& a, int64_t i, double W) {
-    const double x = ((double)a.n * (double)((const T*)a.weights)[i]) / W;
-    return (uint32_t)floor(x) + 1u;
-}
-
 // Per-block totals of the counts.  The block's 16 loads per thread are issued
 // together, raw: a conversion next to each load would serialise the loads.
 template <typename T>
