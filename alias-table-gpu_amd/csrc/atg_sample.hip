@@ -37,18 +37,32 @@ __device__ __forceinline__ void flush_checksum(const SampleArgs& a, uint64_t csu
 
 // ---------------------------------------------------------------------------
 // Basic sampling: one uniform row per sample over the whole table.
+#define BASIC_UNROLL 4
 template <bool AOS>
 __global__ __launch_bounds__(256) void sample_basic(SampleArgs a) {
     const double W_N = a.st->W_N;
     uint64_t csum = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.count; g += stride) {
-        const u32x4 r = philox_at(g, PURPOSE_BASIC, 0, a.seed);
-        const int64_t row = (int64_t)scale64(word64(r), (uint64_t)a.n);
-        double w;
-        int32_t al;
-        load_row<AOS>(a, row, w, al);
-        emit(a, g, decide(w, al, row, coin53(r), W_N, a.n), csum);
+    // BASIC_UNROLL samples per lane and iteration: their row loads are issued
+    // together, so more random reads are in flight (one at a time: 32 ms for
+    // 1e9 samples from 1e8 rows; four: 22 ms; eight: no further gain).
+    for (uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g0 < a.count; g0 += BASIC_UNROLL * stride) {
+        u32x4 r[BASIC_UNROLL];
+        int64_t row[BASIC_UNROLL];
+        double w[BASIC_UNROLL];
+        int32_t al[BASIC_UNROLL];
+#pragma unroll
+        for (int q = 0; q < BASIC_UNROLL; ++q) {
+            const uint64_t g = g0 + (uint64_t)q * stride;
+            r[q] = philox_at(g, PURPOSE_BASIC, 0, a.seed);
+            row[q] = (int64_t)scale64(word64(r[q]), (uint64_t)a.n);
+            if (g < a.count) load_row<AOS>(a, row[q], w[q], al[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < BASIC_UNROLL; ++q) {
+            const uint64_t g = g0 + (uint64_t)q * stride;
+            if (g < a.count) emit(a, g, decide(w[q], al[q], row[q], coin53(r[q]), W_N, a.n), csum);
+        }
     }
     flush_checksum(a, csum);
 }
@@ -379,6 +393,7 @@ __global__ __launch_bounds__(256) void rej_fill_long(RejArgs a) {
 
 // Rejection sampler (SamplerRejection.cu:17-35) with the acceptance fraction in
 // FP64; bounded trials.
+#define REJ_INTERLEAVE 2
 template <typename T>
 __global__ __launch_bounds__(256) void rej_sample(RejArgs a, SampleArgs sa) {
     if (a.st->error) return;
@@ -386,21 +401,48 @@ __global__ __launch_bounds__(256) void rej_sample(RejArgs a, SampleArgs sa) {
     if (Asz == 0 || Asz > a.A_cap) return;
     uint64_t csum = 0, trials = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < sa.count; g += stride) {
-        int32_t item = 0;
-        for (uint32_t tr = 0; tr < REJECTION_MAX_TRIALS; ++tr) {
-            const u32x4 r = philox_at(g, PURPOSE_REJECTION, tr, sa.seed);
-            ++trials;
-            const uint64_t slot = scale64(word64(r), Asz);
-            // One 16-byte read per trial: the first slot of a run accepts with
-            // probability frac(N w / W), every other slot always.
-            const uint4 v = *reinterpret_cast<const uint4*>(&a.A[slot]);
-            item = (int32_t)v.z;
-            const double accept = __longlong_as_double(((long long)v.y << 32) | v.x);
-            if (coin53(r) > accept) continue;
-            break;
+    // REJ_INTERLEAVE samples per lane run their trial loops together, so that
+    // each lane keeps that many random reads of A in flight (one sample at a
+    // time: 4.9 ms per 1e8 samples; two: 4.0 ms; four: no further gain).  Each
+    // sample's trials are those of the one-at-a-time loop.
+    for (uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g0 < sa.count; g0 += REJ_INTERLEAVE * stride) {
+        int32_t it[REJ_INTERLEAVE];
+        uint32_t tq[REJ_INTERLEAVE];
+        bool d[REJ_INTERLEAVE];
+#pragma unroll
+        for (int q = 0; q < REJ_INTERLEAVE; ++q) {
+            it[q] = 0;
+            tq[q] = 0;
+            d[q] = g0 + (uint64_t)q * stride >= sa.count;
         }
-        emit(sa, g, item, csum);
+        while (true) {
+            bool all = true;
+#pragma unroll
+            for (int q = 0; q < REJ_INTERLEAVE; ++q) all = all && d[q];
+            if (all) break;
+            u32x4 r[REJ_INTERLEAVE];
+            uint4 v[REJ_INTERLEAVE];
+#pragma unroll
+            for (int q = 0; q < REJ_INTERLEAVE; ++q) {
+                if (!d[q]) {
+                    r[q] = philox_at(g0 + (uint64_t)q * stride, PURPOSE_REJECTION, tq[q], sa.seed);
+                    v[q] = *reinterpret_cast<const uint4*>(&a.A[scale64(word64(r[q]), Asz)]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < REJ_INTERLEAVE; ++q) {
+                if (!d[q]) {
+                    ++trials;
+                    it[q] = (int32_t)v[q].z;
+                    const double acc = __longlong_as_double(((long long)v[q].y << 32) | v[q].x);
+                    ++tq[q];
+                    d[q] = !(coin53(r[q]) > acc) || tq[q] >= REJECTION_MAX_TRIALS;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < REJ_INTERLEAVE; ++q)
+            if (g0 + (uint64_t)q * stride < sa.count) emit(sa, g0 + (uint64_t)q * stride, it[q], csum);
     }
     flush_checksum(sa, csum);
     if (a.trials) {
