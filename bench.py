@@ -223,7 +223,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard into "
-              f"independent per-GPU tables, see DESIGN.md)", file=sys.stderr)
+              f"independent per-GPU tables)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
