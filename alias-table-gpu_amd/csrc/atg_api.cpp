@@ -126,7 +126,7 @@ struct atg_table_s {
         a.p_req = p_req;
         a.splits = d_splits;
         a.pack_cap = pack_cap_items(n, p_req);
-        a.learned_segments = opts.learned_segments;
+        a.learned_segments = learned_knots(opts.learned_segments, p_req);
         a.knot_j = d_knots;
         return a;
     }

@@ -426,14 +426,14 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         b->pack_cap_max = std::max(b->pack_cap_max, a.pack_cap);
         b->G_max = std::max(b->G_max, z.G);
         a.splits = b->d_splits + osp[t];
-        a.learned_segments = o.learned_segments;
+        a.learned_segments = learned_knots(o.learned_segments, z.p_req);
     }
     // Block->table maps of the phases (blocks per table as the single-table
     // launchers use them).
     auto blocks_of = [&](int ph, int64_t t) -> int64_t {
         const Sizes& z = sz[t];
         const int64_t n = offsets[t + 1] - offsets[t];
-        const int64_t K = std::min<int64_t>(o.learned_segments, z.p_req);
+        const int64_t K = std::min<int64_t>(learned_knots(o.learned_segments, z.p_req), z.p_req);
         switch (ph) {
             case MAP_SUM: return (z.G + 3) / 4;
             case MAP_LOCAL: return z.G;

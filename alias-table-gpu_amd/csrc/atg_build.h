@@ -63,6 +63,15 @@ inline int32_t pack_cap_items(int64_t n, int64_t p_req) {
 // Weight (8 B), item (4 B) and recorded alias (4 B) per pool item.
 inline size_t pack_lds_bytes(int32_t cap) { return (size_t)cap * 16; }
 
+// Knots of the learned split index: at most `opt`, and at most one per 16
+// requested splits (at least 64).  Each knot is a full bisection, so a table
+// with few splits would otherwise bisect nearly every split twice; the
+// splits found are the same for any knot count.
+inline int32_t learned_knots(int32_t opt, int64_t p_req) {
+    const int64_t k = p_req / 16 > 64 ? p_req / 16 : 64;
+    return (int32_t)(k < opt ? k : opt);
+}
+
 size_t psa_lds_bytes(int64_t Ls_max);
 int64_t resident_blocks(const void* f, int threads, size_t lds);
 int64_t block_tree_total(int64_t G);

@@ -59,7 +59,8 @@ typedef struct {
     int64_t num_splits;      /* p (numThreads); 0 = reference AUTO for the chunked pack: 320 * #CU;
                                 negative: one split per -num_splits items (per table in a batch) */
     int32_t split_method;    /* atg_split_method */
-    int32_t learned_segments;/* knots of the learned split index (default 4096) */
+    int32_t learned_segments;/* knots of the learned split index: at most this many (default 4096),
+                                and at most one per 16 splits (at least 64) */
     int32_t record_phases;   /* 1: hipEvents around sum/partition/prefixsum/split/pack */
     int32_t interleave;      /* 1: atg_build runs split and pack interleaved on two streams
                                 (performSplitAndPackInterleaved, AliasTableSplitGpu.cu:220-262);
