@@ -240,6 +240,10 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     ALLOC(t->d_splits, t->p_req + 1);
     ALLOC(t->d_knots, o.learned_segments + 2);
     ALLOC(t->d_checksum, 1);
+    // Section counts of the sectioned sampler at its default 4096-row sections
+    // (smaller sections grow the buffer on first use).
+    t->sec_counts_cap = (n + 4095) / 4096;
+    ALLOC(t->d_sec_counts, 2 * t->sec_counts_cap);
 #undef ALLOC
     if (hipMemset(t->d_state, 0, sizeof(DevState)) != hipSuccess) return cleanup(fail(ATG_EDEVICE, "memset"));
     if (set_partition_lds_limit(psa_lds_bytes(t->Ls)) != hipSuccess)

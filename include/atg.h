@@ -188,9 +188,9 @@ atg_status atg_set_table(atg_table* t, const double* weights, const int32_t* ali
  * built table with Philox4x32-10 (key = seed, counter = output position).
  * out (device, count int32, nullable) receives the items; hist (device, n
  * uint32, nullable) is incremented; checksum (host, nullable) gets the sum of
- * all sampled items.  section_rows: sectioned sampler section (0 = 4096).  The
- * first sectioned call with more sections than any before allocates the
- * table's per-section count buffer (after waiting for the stream). */
+ * all sampled items.  section_rows: sectioned sampler section (0 = 4096).  A
+ * sectioned call with more sections than any before (rows below 4096) grows
+ * the table's per-section count buffer (after waiting for the stream). */
 atg_status atg_sample(atg_table* t, atg_sample_kind kind, uint64_t count, uint64_t seed,
                       int64_t section_rows, int32_t* out, uint32_t* hist, uint64_t* checksum,
                       void* stream);
