@@ -9,11 +9,12 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
+from ._lib import ATG_EINVAL as _EINVAL
 from ._lib import (ATG_AOS, ATG_F32, ATG_F64, ATG_MEM_DEVICE, ATG_MEM_HOST, ATG_SAMPLE_BASIC,
                    ATG_SAMPLE_SECTIONED, ATG_SOA, ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED, DIST, AtgError,
                    BuildOpts, Stats, TableFileInfo, check, lib)
 
-__all__ = ["Table", "Batch", "autotune", "generate_weights", "generate_batch", "default_opts", "lib", "AtgError", "DIST",
+__all__ = ["Table", "Batch", "autotune", "autotune_shape", "generate_weights", "generate_batch", "default_opts", "lib", "AtgError", "DIST",
            "write_table_file", "read_table_file"]
 
 
@@ -25,6 +26,35 @@ def _ptr(a):
     if hasattr(a, "data_ptr"):
         return C.c_void_p(a.data_ptr())
     return a.ctypes.data_as(C.c_void_p)
+
+
+_TORCH_DTYPES = {"f64": "torch.float64", "f32": "torch.float32", "i32": "torch.int32", "u32": "torch.uint32",
+                 "i64": "torch.int64"}
+
+
+def _is_device(x):
+    return hasattr(x, "is_cuda") and bool(x.is_cuda)
+
+
+def _device_buffer(x, kind, numel, name, exact=False):
+    """Validates a device buffer handed to a kernel: a CUDA tensor, contiguous,
+    of element type `kind` ("f64", "f32", "i32", "u32") and at least (exactly,
+    with exact=True) `numel` elements.  None passes through."""
+    if x is None:
+        return None
+    if not _is_device(x):
+        raise AtgError(_EINVAL, f"{name}: a device (torch CUDA) tensor is required, got {type(x).__name__}")
+    if not x.is_contiguous():
+        raise AtgError(_EINVAL, f"{name}: tensor must be contiguous")
+    want = _TORCH_DTYPES[kind]
+    got = str(x.dtype)
+    # hist may be int32 or uint32 (same 4-byte counters)
+    if got != want and not (kind == "u32" and got == _TORCH_DTYPES["i32"]):
+        raise AtgError(_EINVAL, f"{name}: dtype {got}, expected {want}")
+    n = x.numel()
+    if (exact and n != numel) or n < numel:
+        raise AtgError(_EINVAL, f"{name}: {n} elements, {'exactly ' if exact else 'at least '}{numel} required")
+    return x
 
 
 def _stream(s):
@@ -62,24 +92,52 @@ def generate_batch(dist, ntables, lo, hi, seed=42, param=0.0, dtype="f64"):
     return offsets, w
 
 
-def autotune(w, dtype="f64", layout="aos", psa_plus=True, split="bisect", learned_segments=4096, reps=3):
-    """Build shape (section_size, chains, num_splits) chosen by atg_autotune
-    for these weights on this device, and the median build time in ms
-    (negative when it came from the ATG_TUNE_CACHE file)."""
+def _tune_args(w, dtype):
+    if _is_device(w):
+        _device_buffer(w, dtype, w.numel(), "autotune: weights")
+        return _ptr(w), ATG_MEM_DEVICE, w.numel(), w
+    a = np.ascontiguousarray(w, dtype=np.float32 if dtype == "f32" else np.float64)
+    return _ptr(a), ATG_MEM_HOST, a.size, a
+
+
+def _shape_opts(psa_plus, chains, section_size, num_splits, split, learned_segments):
     o = default_opts()
     o.psa_plus = 1 if psa_plus else 0
+    o.chains = int(chains)
+    o.section_size = int(section_size)
+    o.num_splits = int(num_splits)
     o.split_method = ATG_SPLIT_LEARNED if split == "learned" else ATG_SPLIT_BISECT
     o.learned_segments = int(learned_segments)
+    return o
+
+
+def autotune(w, dtype="f64", layout="aos", psa_plus=True, chains=16, section_size=0, num_splits=0, reps=3):
+    """Knobs that leave the table unchanged (split method, learned knots,
+    split/pack interleave) chosen by atg_autotune for these weights at the
+    given build shape, and the median build time in ms (negative when it came
+    from the ATG_TUNE_CACHE file)."""
+    o = _shape_opts(psa_plus, chains, section_size, num_splits, "bisect", 4096)
     best = BuildOpts()
     ms = C.c_double(0.0)
-    if hasattr(w, "is_cuda") and w.is_cuda:
-        ptr, mem, n = _ptr(w), ATG_MEM_DEVICE, w.numel()
-    else:
-        w = np.ascontiguousarray(w, dtype=np.float32 if dtype == "f32" else np.float64)
-        ptr, mem, n = _ptr(w), ATG_MEM_HOST, w.size
+    ptr, mem, n, _keep = _tune_args(w, dtype)
     check(lib().atg_autotune(ptr, mem, int(n), ATG_F32 if dtype == "f32" else ATG_F64,
                              ATG_SOA if layout == "soa" else ATG_AOS, C.byref(o), int(reps), C.byref(best),
                              C.byref(ms)))
+    return {"split": "learned" if best.split_method == ATG_SPLIT_LEARNED else "bisect",
+            "learned_segments": int(best.learned_segments), "interleave": bool(best.interleave)}, ms.value
+
+
+def autotune_shape(w, dtype="f64", layout="aos", psa_plus=True, split="bisect", learned_segments=4096, reps=3):
+    """Build shape (section_size, chains, num_splits) chosen by
+    atg_autotune_shape; the shape changes the table, so it is an explicit
+    input of Table(), and the median build time in ms."""
+    o = _shape_opts(psa_plus, 16, 0, 0, split, learned_segments)
+    best = BuildOpts()
+    ms = C.c_double(0.0)
+    ptr, mem, n, _keep = _tune_args(w, dtype)
+    check(lib().atg_autotune_shape(ptr, mem, int(n), ATG_F32 if dtype == "f32" else ATG_F64,
+                                   ATG_SOA if layout == "soa" else ATG_AOS, C.byref(o), int(reps), C.byref(best),
+                                   C.byref(ms)))
     return {"section_size": int(best.section_size), "chains": int(best.chains), "num_splits": int(best.num_splits),
             "split": split, "psa_plus": psa_plus}, ms.value
 
@@ -146,13 +204,15 @@ class Table:
 
     # -- construction ---------------------------------------------------
     def set_weights(self, w, stream=None):
-        if hasattr(w, "is_cuda") and w.is_cuda:
+        if _is_device(w):
+            _device_buffer(w, self.dtype, self.n, "set_weights: weights", exact=True)
             self._keep = w
             check(lib().atg_set_weights(self.h, _ptr(w), ATG_MEM_DEVICE, _stream(stream)))
         else:
             want = np.float32 if self.dtype == "f32" else np.float64
             a = np.ascontiguousarray(w, dtype=want)
-            assert a.size == self.n
+            if a.size != self.n:
+                raise AtgError(_EINVAL, f"set_weights: {a.size} weights for a table of {self.n}")
             check(lib().atg_set_weights(self.h, _ptr(a), ATG_MEM_HOST, _stream(stream)))
 
     def build(self, stream=None):
@@ -190,13 +250,16 @@ class Table:
 
     def set_table(self, w, alias, W, stream=None):
         """Loads rows (e.g. from read_table_file) for sampling (atg_set_table)."""
-        if hasattr(w, "is_cuda") and w.is_cuda:
+        if _is_device(w) or _is_device(alias):
+            _device_buffer(w, "f64", self.n, "set_table: weights", exact=True)
+            _device_buffer(alias, "i32", self.n, "set_table: aliases", exact=True)
             self._keep = (w, alias)
             check(lib().atg_set_table(self.h, _ptr(w), _ptr(alias), float(W), ATG_MEM_DEVICE, _stream(stream)))
         else:
             wa = np.ascontiguousarray(w, dtype=np.float64)
             aa = np.ascontiguousarray(alias, dtype=np.int32)
-            assert wa.size == aa.size == self.n
+            if not wa.size == aa.size == self.n:
+                raise AtgError(_EINVAL, f"set_table: {wa.size} weights and {aa.size} aliases for a table of {self.n}")
             check(lib().atg_set_table(self.h, _ptr(wa), _ptr(aa), float(W), ATG_MEM_HOST, _stream(stream)))
 
     def device_table_ptr(self):
@@ -224,7 +287,11 @@ class Table:
     # -- sampling ----------------------------------------------------------
     def sample(self, count, seed, kind="sectioned", section_rows=4096, out=None, hist=None, checksum=False,
                stream=None):
+        if kind not in ("sectioned", "basic"):
+            raise AtgError(_EINVAL, f"sample: unknown kind {kind!r}")
         k = ATG_SAMPLE_SECTIONED if kind == "sectioned" else ATG_SAMPLE_BASIC
+        _device_buffer(out, "i32", int(count), "sample: out")
+        _device_buffer(hist, "u32", self.n, "sample: hist")
         cs = C.c_uint64(0)
         check(lib().atg_sample(self.h, k, int(count), int(seed), int(section_rows), _ptr(out), _ptr(hist),
                                C.byref(cs) if checksum else None, _stream(stream)))
@@ -245,6 +312,8 @@ class Table:
         return A
 
     def rejection_sample(self, count, seed, out=None, hist=None, checksum=False, stream=None):
+        _device_buffer(out, "i32", int(count), "rejection_sample: out")
+        _device_buffer(hist, "u32", self.n, "rejection_sample: hist")
         cs = C.c_uint64(0)
         check(lib().atg_rejection_sample(self.h, int(count), int(seed), _ptr(out), _ptr(hist),
                                          C.byref(cs) if checksum else None, _stream(stream)))
@@ -258,8 +327,9 @@ class Batch:
     `Table` with the same options would build it alone."""
 
     def __init__(self, offsets, dtype="f64", psa_plus=True, chains=16, section_size=0, num_splits=0,
-                 split="bisect", learned_segments=4096):
+                 split="bisect", learned_segments=4096, group_items=0):
         o = default_opts()
+        o.batch_group_items = int(group_items)
         o.psa_plus = 1 if psa_plus else 0
         o.chains = int(chains)
         o.section_size = int(section_size)
@@ -288,13 +358,15 @@ class Batch:
             pass
 
     def set_weights(self, w, stream=None):
-        if hasattr(w, "is_cuda") and w.is_cuda:
+        if _is_device(w):
+            _device_buffer(w, self.dtype, self.total, "Batch.set_weights: weights", exact=True)
             self._keep = w
             check(lib().atg_batch_set_weights(self.h, _ptr(w), ATG_MEM_DEVICE, _stream(stream)))
         else:
             want = np.float32 if self.dtype == "f32" else np.float64
             a = np.ascontiguousarray(w, dtype=want)
-            assert a.size == self.total
+            if a.size != self.total:
+                raise AtgError(_EINVAL, f"Batch.set_weights: {a.size} weights for {self.total} items")
             check(lib().atg_batch_set_weights(self.h, _ptr(a), ATG_MEM_HOST, _stream(stream)))
 
     def build(self, stream=None):
@@ -315,10 +387,40 @@ class Batch:
         check(lib().atg_batch_get_tables(self.h, _ptr(w), _ptr(a), _ptr(W)))
         return w, a, W
 
+    def table(self, t):
+        """(row weights, aliases, W) of table t alone (atg_batch_get_table)."""
+        n = int(self.offsets[t + 1] - self.offsets[t])
+        w = np.empty(n, np.float64)
+        a = np.empty(n, np.int32)
+        W = C.c_double(0.0)
+        check(lib().atg_batch_get_table(self.h, int(t), _ptr(w), _ptr(a), C.byref(W)))
+        return w, a, W.value
+
+    def groups(self):
+        """Group of every table (the launch groups of batch_group_items)."""
+        g = np.empty(self.ntables, np.int32)
+        ng = C.c_int64(0)
+        check(lib().atg_batch_groups(self.h, _ptr(g), C.byref(ng)))
+        return g, ng.value
+
     def sample(self, per_table, seed, out, stream=None):
         """per_table samples of every table into the device buffer `out`
         (int32, ntables * per_table; item indices local to each table)."""
+        if out is None:
+            raise AtgError(_EINVAL, "Batch.sample: out is required")
+        _device_buffer(out, "i32", self.ntables * int(per_table), "Batch.sample: out")
         check(lib().atg_batch_sample(self.h, int(per_table), int(seed), _ptr(out), _stream(stream)))
+
+    def autotune(self, reps=3, makespan_slack=0.05):
+        """Tunes the split method and the launch groups in place
+        (atg_batch_autotune); returns the chosen split and group size, the
+        median makespan and the 99th-percentile table latency in ms."""
+        o = BuildOpts()
+        ms, p99 = C.c_double(0.0), C.c_double(0.0)
+        check(lib().atg_batch_autotune(self.h, int(reps), float(makespan_slack), C.byref(o), C.byref(ms),
+                                       C.byref(p99)))
+        return {"split": "learned" if o.split_method == ATG_SPLIT_LEARNED else "bisect",
+                "group_items": int(o.batch_group_items), "makespan_ms": ms.value, "p99_latency_ms": p99.value}
 
     def latencies_ns(self):
         v = np.empty(self.ntables, np.float64)

@@ -28,7 +28,7 @@ DIST = {
 class BuildOpts(C.Structure):
     _fields_ = [("psa_plus", C.c_int32), ("chains", C.c_int32), ("section_size", C.c_int64),
                 ("num_splits", C.c_int64), ("split_method", C.c_int32), ("learned_segments", C.c_int32),
-                ("record_phases", C.c_int32), ("interleave", C.c_int32)]
+                ("record_phases", C.c_int32), ("interleave", C.c_int32), ("batch_group_items", C.c_int64)]
 
 
 class Stats(C.Structure):
@@ -77,12 +77,17 @@ SIGNATURES = [
     ("atg_generate_weights", _I32, [_I32, _I64, _U64, _D, _P]),
     ("atg_generate_batch", _I32, [_I32, _I64, _I64, _I64, _U64, _D, _P, _P, _I32]),
     ("atg_autotune", _I32, [_P, _I32, _I64, _I32, _I32, C.POINTER(BuildOpts), _I32, C.POINTER(BuildOpts), C.POINTER(_D)]),
+    ("atg_autotune_shape", _I32, [_P, _I32, _I64, _I32, _I32, C.POINTER(BuildOpts), _I32, C.POINTER(BuildOpts),
+                                  C.POINTER(_D)]),
+    ("atg_batch_autotune", _I32, [_P, _I32, _D, C.POINTER(BuildOpts), C.POINTER(_D), C.POINTER(_D)]),
     ("atg_batch_create", _I32, [C.POINTER(_P), _P, _I64, _I32, C.POINTER(BuildOpts)]),
     ("atg_batch_destroy", None, [_P]),
     ("atg_batch_set_weights", _I32, [_P, _P, _I32, _P]),
     ("atg_batch_build", _I32, [_P, _P]),
     ("atg_batch_sync", _I32, [_P, _P]),
     ("atg_batch_get_tables", _I32, [_P, _P, _P, _P]),
+    ("atg_batch_get_table", _I32, [_P, _I64, _P, _P, C.POINTER(_D)]),
+    ("atg_batch_groups", _I32, [_P, _P, C.POINTER(_I64)]),
     ("atg_batch_sample", _I32, [_P, _U64, _U64, _P, _P]),
     ("atg_batch_latencies", _I32, [_P, _P]),
     ("atg_batch_use_graph", _I32, [_P, _I32]),

@@ -59,6 +59,7 @@ struct atg_table_s {
     atg_build_opts opts{};
     int64_t G = 0, Ls = 0, tree_total = 0, p_req = 0, tree_half = 0;
     int64_t p_cap = 0;  // capacity of the splits array minus one
+    int32_t knots_cap = 0;  // learned-index knots the knot array holds
     int device = 0;
     // device memory
     void* d_weights_owned = nullptr;
@@ -78,7 +79,8 @@ struct atg_table_s {
     Split* d_splits = nullptr;
     int32_t* d_knots = nullptr;
     unsigned long long* d_checksum = nullptr;
-    // rejection
+    // rejection: its own W and |A| (the alias build's state stays untouched)
+    DevState* d_rej_state = nullptr;
     RejEntry* d_A = nullptr;
     uint64_t A_cap = 0;
     uint64_t *d_rej_tot = nullptr, *d_rej_off = nullptr;
@@ -90,6 +92,9 @@ struct atg_table_s {
     // host side
     std::vector<double> h_weights;  // host copy when weights were given on the host
     bool weights_set = false, built = false;
+    // Stream of the last work enqueued on this table; the host-side getters
+    // wait for it (not for the whole device) before reading device state.
+    hipStream_t last_stream = nullptr;
     hipEvent_t ev[6] = {};
     bool events = false;
     // Split/pack interleave: the splits run in chunks on sp_stream, each pack
@@ -159,7 +164,8 @@ static void free_table(atg_table* t) {
     void* ptrs[] = {t->d_weights_owned, t->d_state, t->d_section_sums, t->d_tree, t->d_slots, t->d_sec_cert, t->d_stage_w,
                     t->d_stage_i, t->d_Li, t->d_Hi,
                     t->d_Lw, t->d_Lp, t->d_Hw, t->d_Hp, t->d_rows, t->d_tw, t->d_ta, t->d_splits, t->d_knots,
-                    t->d_checksum, t->d_A, t->d_rej_tot, t->d_rej_off, t->d_rej_tasks, t->d_sec_counts};
+                    t->d_checksum, t->d_rej_state, t->d_A, t->d_rej_tot, t->d_rej_off, t->d_rej_tasks,
+                    t->d_sec_counts};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->events)
@@ -239,6 +245,7 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     t->p_cap = t->p_req;
     ALLOC(t->d_splits, t->p_req + 1);
     ALLOC(t->d_knots, o.learned_segments + 2);
+    t->knots_cap = o.learned_segments;
     ALLOC(t->d_checksum, 1);
     // Section counts of the sectioned sampler at its default 4096-row sections
     // (smaller sections grow the buffer on first use).
@@ -290,6 +297,7 @@ atg_status atg_set_weights(atg_table* t, const void* w, atg_mem mem, void* strea
     t->weights_set = true;
     t->built = false;
     t->rej_built = false;
+    t->last_stream = s;
     return ATG_OK;
 }
 
@@ -297,6 +305,7 @@ atg_status atg_build_phase(atg_table* t, int32_t phase, void* stream) {
     if (!t) return fail(ATG_EINVAL, "atg_build_phase: NULL table");
     if (!t->weights_set) return fail(ATG_ESTATE, "atg_build_phase: weights not set");
     hipStream_t s = (hipStream_t)stream;
+    t->last_stream = s;
     const BuildArgs a = t->args();
     switch (phase) {
         case ATG_PHASE_SUM: HIPCHK(launch_sum(a, s)); break;
@@ -347,7 +356,8 @@ atg_status atg_build(atg_table* t, void* stream) {
     if (!t) return fail(ATG_EINVAL, "atg_build: NULL table");
     if (!t->weights_set) return fail(ATG_ESTATE, "atg_build: weights not set");
     hipStream_t s = (hipStream_t)stream;
-    if (t->sp_stream && !t->events) {
+    t->last_stream = s;
+    if (t->opts.interleave && t->sp_stream && !t->events) {
         const BuildArgs a = t->args();
         HIPCHK(launch_sum(a, s));
         HIPCHK(launch_partition(a, s));
@@ -384,7 +394,7 @@ atg_status atg_sync(atg_table* t, void* stream) {
 
 atg_status atg_get_stats(atg_table* t, atg_stats* out) {
     if (!t || !out) return fail(ATG_EINVAL, "atg_get_stats: NULL argument");
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipStreamSynchronize(t->last_stream));
     DevState st;
     HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
     std::memset(out, 0, sizeof(*out));
@@ -493,6 +503,7 @@ atg_status atg_set_table(atg_table* t, const double* w, const int32_t* al, doubl
     HIPCHK(hipStreamSynchronize(s));
     t->built = true;
     t->rej_built = false;
+    t->last_stream = s;
     return ATG_OK;
 }
 
@@ -511,7 +522,7 @@ atg_status atg_table_device_ptr(atg_table* t, void** a, void** b) {
 atg_status atg_get_splits(atg_table* t, int32_t* i, int32_t* j, double* spill, int64_t cap) {
     if (!t || !i || !j || !spill) return fail(ATG_EINVAL, "atg_get_splits: NULL argument");
     DevState st;
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipStreamSynchronize(t->last_stream));
     HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
     if (st.p + 1 > cap) return fail(ATG_EINVAL, "atg_get_splits: capacity %lld < p+1=%lld", (long long)cap, (long long)(st.p + 1));
     std::vector<Split> sp(st.p + 1);
@@ -527,7 +538,7 @@ atg_status atg_get_splits(atg_table* t, int32_t* i, int32_t* j, double* spill, i
 atg_status atg_get_lh(atg_table* t, int32_t* Li, double* Lw, double* Lp, int32_t* Hi, double* Hw, double* Hp) {
     if (!t) return fail(ATG_EINVAL, "atg_get_lh: NULL table");
     DevState st;
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipStreamSynchronize(t->last_stream));
     HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
     if (Li) HIPCHK(hipMemcpy(Li, t->d_Li, 4 * st.numL, hipMemcpyDeviceToHost));
     if (Lw) HIPCHK(hipMemcpy(Lw, t->d_Lw, 8 * st.numL, hipMemcpyDeviceToHost));
@@ -604,6 +615,7 @@ atg_status atg_sample(atg_table* t, atg_sample_kind kind, uint64_t count, uint64
     if (!t) return fail(ATG_EINVAL, "atg_sample: NULL table");
     if (!t->built) return fail(ATG_ESTATE, "atg_sample: table not built");
     hipStream_t s = (hipStream_t)stream;
+    t->last_stream = s;
     SampleArgs a{};
     a.rows = t->d_rows;
     a.tw = t->d_tw;
@@ -658,23 +670,40 @@ atg_status atg_rejection_build(atg_table* t, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     const int64_t nb = rejection_blocks(t->n);
     if (!t->d_A) {
-        t->A_cap = 2 * (uint64_t)t->n + 4096;
-        if (dalloc(&t->d_A, t->A_cap) != hipSuccess) { (void)hipGetLastError(); return fail(ATG_ENOMEM, "rejection A"); }
-        if (dalloc(&t->d_rej_tot, nb) != hipSuccess) { (void)hipGetLastError(); return fail(ATG_ENOMEM, "rejection totals"); }
-        if (dalloc(&t->d_rej_off, nb) != hipSuccess) { (void)hipGetLastError(); return fail(ATG_ENOMEM, "rejection offsets"); }
-        if (dalloc(&t->d_rej_tasks, rejection_task_cap(t->A_cap)) != hipSuccess) {
+        // All five buffers or none: a partial set would leave later calls
+        // launching on null pointers.
+        const uint64_t cap = 2 * (uint64_t)t->n + 4096;
+        DevState* st = nullptr;
+        RejEntry* A = nullptr;
+        uint64_t *tot = nullptr, *off = nullptr;
+        RejTask* tasks = nullptr;
+        if (dalloc(&st, 1) != hipSuccess || dalloc(&A, cap) != hipSuccess || dalloc(&tot, nb) != hipSuccess ||
+            dalloc(&off, nb) != hipSuccess || dalloc(&tasks, rejection_task_cap(cap)) != hipSuccess ||
+            hipMemsetAsync(st, 0, sizeof(DevState), s) != hipSuccess) {
             (void)hipGetLastError();
-            return fail(ATG_ENOMEM, "rejection tasks");
+            void* ptrs[] = {st, A, tot, off, tasks};
+            for (void* q : ptrs)
+                if (q) (void)hipFree(q);
+            return fail(ATG_ENOMEM, "atg_rejection_build: device allocation of the rejection structure failed");
         }
+        t->d_rej_state = st;
+        t->d_A = A;
+        t->A_cap = cap;
+        t->d_rej_tot = tot;
+        t->d_rej_off = off;
+        t->d_rej_tasks = tasks;
     }
-    // W: the canonical sum (RejectionSampling.cu:36 uses the host sum).
+    t->last_stream = s;
+    // W: the canonical sum (RejectionSampling.cu:36 uses the host sum), into
+    // the rejection structure's own state.
     BuildArgs ba = t->args();
+    ba.st = t->d_rej_state;
     HIPCHK(launch_sum(ba, s));
     RejArgs r{};
     r.weights = t->d_weights;
     r.dtype_f32 = t->dtype == ATG_F32;
     r.n = t->n;
-    r.st = t->d_state;
+    r.st = t->d_rej_state;
     r.nb = nb;
     r.block_tot = t->d_rej_tot;
     r.block_off = t->d_rej_off;
@@ -689,9 +718,10 @@ atg_status atg_rejection_build(atg_table* t, void* stream) {
 
 atg_status atg_rejection_size(atg_table* t, uint64_t* a_size) {
     if (!t || !a_size) return fail(ATG_EINVAL, "atg_rejection_size: NULL argument");
+    if (!t->rej_built) return fail(ATG_ESTATE, "atg_rejection_size: rejection structure not built");
     DevState st;
-    HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(hipStreamSynchronize(t->last_stream));
+    HIPCHK(hipMemcpy(&st, t->d_rej_state, sizeof(st), hipMemcpyDeviceToHost));
     *a_size = st.rej_size;
     return state_status(st);
 }
@@ -712,15 +742,16 @@ atg_status atg_rejection_sample(atg_table* t, uint64_t count, uint64_t seed, int
     if (!t) return fail(ATG_EINVAL, "atg_rejection_sample: NULL table");
     if (!t->rej_built) return fail(ATG_ESTATE, "atg_rejection_sample: rejection structure not built");
     hipStream_t s = (hipStream_t)stream;
+    t->last_stream = s;
     RejArgs r{};
     r.weights = t->d_weights;
     r.dtype_f32 = t->dtype == ATG_F32;
     r.n = t->n;
-    r.st = t->d_state;
+    r.st = t->d_rej_state;
     r.A = t->d_A;
     r.A_cap = t->A_cap;
     SampleArgs a{};
-    a.st = t->d_state;
+    a.st = t->d_rej_state;
     a.n = t->n;
     a.count = count;
     a.seed = seed;
@@ -753,6 +784,28 @@ atg_status table_set_shape(atg_table* t, int32_t chains, int64_t num_splits) {
     t->opts.chains = chains;
     t->opts.num_splits = num_splits;
     t->p_req = p;
+    t->built = false;
+    return ATG_OK;
+}
+
+// Changes the knobs of a created table that leave its rows unchanged: the
+// split method (the learned index returns the bisection's splits), the knot
+// count (at most the count the table was created with) and the split/pack
+// interleave (its second stream is created on first use).
+atg_status table_set_method(atg_table* t, int32_t split_method, int32_t learned_segments, int32_t interleave) {
+    if (!t) return fail(ATG_EINVAL, "table_set_method: NULL table");
+    if (split_method != ATG_SPLIT_BISECT && split_method != ATG_SPLIT_LEARNED)
+        return fail(ATG_EINVAL, "table_set_method: unknown split method %d", split_method);
+    if (learned_segments < 1 || learned_segments > t->knots_cap)
+        return fail(ATG_EINVAL, "table_set_method: %d knots outside [1, %d]", learned_segments, t->knots_cap);
+    if (interleave && !t->sp_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&t->sp_stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&t->ev_fork, hipEventDisableTiming));
+        for (auto& e : t->ev_chunk) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    t->opts.split_method = split_method;
+    t->opts.learned_segments = learned_segments;
+    t->opts.interleave = interleave ? 1 : 0;
     t->built = false;
     return ATG_OK;
 }

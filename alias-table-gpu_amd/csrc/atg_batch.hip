@@ -1,13 +1,17 @@
 // Batched construction of many independent alias tables (no counterpart in
 // the reference, which builds one table per object: AliasTableSplitGpu.cu:388-468).
 //
-// Every phase is one launch over all tables: a block finds its table and its
-// block within the table from a block->table map built at creation, then runs
-// exactly the device code of the single-table kernels (the same launch shape
-// per table: one wave per section, one wave per 64 pack chains).  A table built here is therefore bit-identical to the same weights
-// built alone with the same options.  Per-table latencies come from the
-// 100 MHz device clock: the first block of the table's sum phase to the last
-// block of its pack.
+// The tables are dealt into groups (largest table first, each to the group
+// with the fewest items so far; one group unless batch_group_items is set).
+// Every phase of a group is one launch over the group's tables: a block finds
+// its table and its block within the table from a block->table map built at
+// creation, then runs exactly the device code of the single-table kernels
+// (the same launch shape per table: one wave per section, one wave per 64
+// pack chains).  A table built here is therefore bit-identical to the same
+// weights built alone with the same options.  The groups run one after the
+// other, so a table is done when its group is.  Per-table latencies come from
+// the 100 MHz device clock: the first block of the table's sum phase to the
+// last block of its pack.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -85,8 +89,9 @@ __global__ __launch_bounds__(256) void seg_sum(SegArgs g, SegMap m) {
     sum_sections_block<T>(a, blk, (a.G + 3) / 4);
 }
 
-__global__ __launch_bounds__(256) void seg_sum_finish(SegArgs g) {
-    sum_finish_block(g.tables[blockIdx.x]);
+// One block per table of the group.
+__global__ __launch_bounds__(256) void seg_sum_finish(SegArgs g, SegMap m) {
+    sum_finish_block(g.tables[m.tab[blockIdx.x]]);
 }
 
 // Sections of all tables, grid-stride over the phase's blocks (one section
@@ -149,9 +154,9 @@ __global__ __launch_bounds__(256) void seg_split(SegArgs g, SegMap m) {
     split_bisect_block(g.tables[t], blk);
 }
 
-// The split certificate of every table: one block per table.
-__global__ __launch_bounds__(256) void seg_cert(SegArgs g) {
-    const BuildArgs& a = g.tables[blockIdx.x];
+// The split certificate of every table of the group: one block per table.
+__global__ __launch_bounds__(256) void seg_cert(SegArgs g, SegMap m) {
+    const BuildArgs& a = g.tables[m.tab[blockIdx.x]];
     if (a.st->error) return;
     cert_reduce_block(a, 0, 1);
 }
@@ -216,7 +221,16 @@ __global__ __launch_bounds__(256) void batch_sample(BatchSampleArgs a) {
 
 using namespace atg;
 
-enum { MAP_SUM, MAP_LOCAL, MAP_PLACE, MAP_DEG, MAP_SPLIT, MAP_KNOTS, MAP_PACK, MAP_TREE, MAP_COUNT };
+enum { MAP_SUM, MAP_LOCAL, MAP_PLACE, MAP_DEG, MAP_SPLIT, MAP_KNOTS, MAP_PACK, MAP_TREE, MAP_ONE, MAP_COUNT };
+
+// The launches of one group: a block->table map per phase (MAP_ONE: one block
+// per table) and the group's most sections of a table (tree levels).
+struct BatchGroup {
+    SegMap maps[MAP_COUNT];
+    int64_t G_max = 1;
+    int64_t items = 0;
+    int64_t ntables = 0;
+};
 
 struct atg_batch_s {
     int64_t ntables = 0, total = 0;
@@ -227,9 +241,9 @@ struct atg_batch_s {
     std::vector<BuildArgs> args;        // host copy (pointers into the pools below)
     int64_t Ls_max = 1;
     int32_t pack_cap_max = 16;          // largest pack LDS pool of the tables (items)
-    int64_t G_max = 1;                  // most sections of a table
     double wall_khz = 100000.0;
-    SegMap maps[MAP_COUNT] = {};
+    std::vector<BatchGroup> groups;     // launched in order
+    std::vector<int32_t> group_of;      // table -> group
     // device memory
     void* d_weights_owned = nullptr;
     const void* d_weights = nullptr;
@@ -242,9 +256,10 @@ struct atg_batch_s {
     uint64_t* d_slots = nullptr;
     int32_t* d_ipool = nullptr;         // L/H items, staging, knots
     Split* d_splits = nullptr;
-    int32_t* d_maptab = nullptr;        // block->table maps of all phases
-    int64_t* d_mapbase = nullptr;       // first block of every table, all phases
+    int32_t* d_maptab = nullptr;        // block->table maps of all phases and groups
+    int64_t* d_mapbase = nullptr;       // first block of every table within its group's map, all phases
     bool weights_set = false, built = false;
+    hipStream_t last_stream = nullptr;  // stream of the last build (the getters wait for it)
     // The build's launch sequence captured once and replayed (atg_batch_use_graph).
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -282,30 +297,146 @@ void bind_weights(atg_batch* b) {
 template <typename T>
 hipError_t launch_batch(const atg_batch* b, hipStream_t s) {
     SegArgs g{b->d_args, b->ntables, b->d_ticks};
-    const SegMap* M = b->maps;
     auto grid = [](const SegMap& m) { return dim3((unsigned)m.blocks); };
     hipLaunchKernelGGL(seg_reset_ticks, dim3((unsigned)std::min<int64_t>((b->ntables + 255) / 256, 4096)), dim3(256),
                        0, s, g);
-    hipLaunchKernelGGL(seg_sum<T>, grid(M[MAP_SUM]), dim3(256), 0, s, g, M[MAP_SUM]);
-    hipLaunchKernelGGL(seg_sum_finish, dim3((unsigned)b->ntables), dim3(256), 0, s, g);
     const size_t lds = psa_lds_bytes(b->Ls_max);
     const int64_t resident = resident_blocks((const void*)seg_local<T>, 64, lds);
-    hipLaunchKernelGGL(seg_local<T>, dim3((unsigned)std::min<int64_t>(M[MAP_LOCAL].blocks, resident)), dim3(64), lds,
-                       s, g, M[MAP_LOCAL]);
-    for (int b0 = 0; (b->G_max >> b0) > 1; b0 += 9)
-        hipLaunchKernelGGL(seg_tree, grid(M[MAP_TREE]), dim3(256), 0, s, g, M[MAP_TREE], b0);
-    hipLaunchKernelGGL(seg_place, grid(M[MAP_PLACE]), dim3(256), 0, s, g, M[MAP_PLACE]);
-    if (b->opts.split_method == ATG_SPLIT_LEARNED)
-        hipLaunchKernelGGL(seg_cert, dim3((unsigned)b->ntables), dim3(256), 0, s, g);
-    hipLaunchKernelGGL(seg_degenerate, grid(M[MAP_DEG]), dim3(256), 0, s, g, M[MAP_DEG]);
-    if (b->opts.split_method == ATG_SPLIT_LEARNED) {
-        hipLaunchKernelGGL(seg_knots, grid(M[MAP_KNOTS]), dim3(256), 0, s, g, M[MAP_KNOTS]);
-        hipLaunchKernelGGL(seg_learned, grid(M[MAP_SPLIT]), dim3(256), 0, s, g, M[MAP_SPLIT]);
-    } else {
-        hipLaunchKernelGGL(seg_split, grid(M[MAP_SPLIT]), dim3(256), 0, s, g, M[MAP_SPLIT]);
+    const bool learned = b->opts.split_method == ATG_SPLIT_LEARNED;
+    for (const BatchGroup& grp : b->groups) {
+        const SegMap* M = grp.maps;
+        // Every group holds at least one table, so each map below has blocks
+        // (MAP_KNOTS and MAP_TREE are checked: a group of one-section tables
+        // has no tree blocks).
+        hipLaunchKernelGGL(seg_sum<T>, grid(M[MAP_SUM]), dim3(256), 0, s, g, M[MAP_SUM]);
+        hipLaunchKernelGGL(seg_sum_finish, grid(M[MAP_ONE]), dim3(256), 0, s, g, M[MAP_ONE]);
+        hipLaunchKernelGGL(seg_local<T>, dim3((unsigned)std::min<int64_t>(M[MAP_LOCAL].blocks, resident)), dim3(64),
+                           lds, s, g, M[MAP_LOCAL]);
+        if (M[MAP_TREE].blocks > 0)
+            for (int b0 = 0; (grp.G_max >> b0) > 1; b0 += 9)
+                hipLaunchKernelGGL(seg_tree, grid(M[MAP_TREE]), dim3(256), 0, s, g, M[MAP_TREE], b0);
+        hipLaunchKernelGGL(seg_place, grid(M[MAP_PLACE]), dim3(256), 0, s, g, M[MAP_PLACE]);
+        if (learned) hipLaunchKernelGGL(seg_cert, grid(M[MAP_ONE]), dim3(256), 0, s, g, M[MAP_ONE]);
+        hipLaunchKernelGGL(seg_degenerate, grid(M[MAP_DEG]), dim3(256), 0, s, g, M[MAP_DEG]);
+        if (learned) {
+            if (M[MAP_KNOTS].blocks > 0)
+                hipLaunchKernelGGL(seg_knots, grid(M[MAP_KNOTS]), dim3(256), 0, s, g, M[MAP_KNOTS]);
+            hipLaunchKernelGGL(seg_learned, grid(M[MAP_SPLIT]), dim3(256), 0, s, g, M[MAP_SPLIT]);
+        } else {
+            hipLaunchKernelGGL(seg_split, grid(M[MAP_SPLIT]), dim3(256), 0, s, g, M[MAP_SPLIT]);
+        }
+        hipLaunchKernelGGL(seg_pack, grid(M[MAP_PACK]), dim3(64), pack_lds_bytes(b->pack_cap_max), s, g, M[MAP_PACK]);
     }
-    hipLaunchKernelGGL(seg_pack, grid(M[MAP_PACK]), dim3(64), pack_lds_bytes(b->pack_cap_max), s, g, M[MAP_PACK]);
     return hipGetLastError();
+}
+
+// Deals the tables into groups (largest table first, each to the group with
+// the fewest items so far, ties to the lower group; members keep their index
+// order) and uploads the block->table maps of every group and phase.  Blocks
+// per table as the single-table launchers use them.  Replaces any earlier
+// grouping; the caller drops a captured graph.
+atg_status make_groups(atg_batch* b, int64_t group_items) {
+    const int64_t ntables = b->ntables;
+    const std::vector<int64_t>& offsets = b->offsets;
+    int64_t ngroups = 1;
+    if (group_items > 0)
+        ngroups = std::min<int64_t>(ntables, std::max<int64_t>(1, (b->total + group_items - 1) / group_items));
+    std::vector<BatchGroup> groups((size_t)ngroups);
+    std::vector<int32_t> group_of((size_t)ntables, 0);
+    std::vector<std::vector<int64_t>> members((size_t)ngroups);
+    {
+        std::vector<int64_t> order((size_t)ntables);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+            return offsets[x + 1] - offsets[x] > offsets[y + 1] - offsets[y];
+        });
+        for (int64_t t : order) {
+            int64_t g = 0;
+            for (int64_t q = 1; q < ngroups; ++q)
+                if (groups[q].items < groups[g].items) g = q;
+            groups[g].items += offsets[t + 1] - offsets[t];
+            groups[g].ntables += 1;
+            group_of[t] = (int32_t)g;
+        }
+        for (int64_t t = 0; t < ntables; ++t) members[group_of[t]].push_back(t);
+        for (int64_t g = 0; g < ngroups; ++g)
+            for (int64_t t : members[g]) groups[g].G_max = std::max(groups[g].G_max, b->args[t].G);
+    }
+    auto blocks_of = [&](int ph, int64_t t) -> int64_t {
+        const BuildArgs& a = b->args[t];
+        const int64_t K = std::min<int64_t>(a.learned_segments, a.p_req);
+        switch (ph) {
+            case MAP_SUM: return (a.G + 3) / 4;
+            case MAP_LOCAL: return a.G;
+            case MAP_PLACE: return (a.G + 3) / 4;
+            case MAP_TREE: return a.G > 1 ? (a.G + 511) / 512 : 0;  // one section: no tree
+            case MAP_DEG: return degenerate_blocks(a.n);
+            case MAP_SPLIT: return std::max<int64_t>(1, (a.p_req - 1 + 255) / 256);
+            case MAP_KNOTS: return (K + 1 + 255) / 256;
+            case MAP_ONE: return 1;
+            default: return (a.p_req + 63) / 64;  // MAP_PACK
+        }
+    };
+    // The maps of phase ph, group g follow each other (phase-major); base
+    // holds each table's first block relative to its group's map.
+    std::vector<int64_t> base((size_t)MAP_COUNT * ntables);
+    std::vector<int32_t> tab;
+    std::vector<int64_t> map_start((size_t)MAP_COUNT * ngroups);
+    for (int ph = 0; ph < MAP_COUNT; ++ph)
+        for (int64_t g = 0; g < ngroups; ++g) {
+            map_start[(size_t)ph * ngroups + g] = (int64_t)tab.size();
+            int64_t nblk = 0;
+            for (int64_t t : members[g]) {
+                base[(size_t)ph * ntables + t] = nblk;
+                const int64_t nb = blocks_of(ph, t);
+                tab.insert(tab.end(), (size_t)nb, (int32_t)t);
+                nblk += nb;
+            }
+            if (nblk > 0x7FFFFFFF) return fail(ATG_EINVAL, "batch: %lld blocks in one launch", (long long)nblk);
+            groups[g].maps[ph].blocks = nblk;
+        }
+    int32_t* d_tab = nullptr;
+    int64_t* d_base = nullptr;
+    if (dalloc(&d_tab, tab.size()) != hipSuccess || dalloc(&d_base, base.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        if (d_tab) (void)hipFree(d_tab);
+        return fail(ATG_ENOMEM, "batch: block maps");
+    }
+    if (hipMemcpy(d_tab, tab.data(), sizeof(int32_t) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_base, base.data(), sizeof(int64_t) * base.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d_tab);
+        (void)hipFree(d_base);
+        return fail(ATG_EDEVICE, "batch: block map upload");
+    }
+    for (int ph = 0; ph < MAP_COUNT; ++ph)
+        for (int64_t g = 0; g < ngroups; ++g) {
+            SegMap& m = groups[g].maps[ph];
+            m.tab = d_tab + map_start[(size_t)ph * ngroups + g];
+            m.base = d_base + (size_t)ph * ntables;
+        }
+    // The old maps may still be read by enqueued launches.
+    if (b->d_maptab || b->d_mapbase) (void)hipDeviceSynchronize();
+    if (b->d_maptab) (void)hipFree(b->d_maptab);
+    if (b->d_mapbase) (void)hipFree(b->d_mapbase);
+    b->d_maptab = d_tab;
+    b->d_mapbase = d_base;
+    b->groups.swap(groups);
+    b->group_of.swap(group_of);
+    b->opts.batch_group_items = group_items > 0 ? group_items : 0;
+    return ATG_OK;
+}
+
+// Switches every table between bisection and the learned split (the same
+// splits either way; the learned split also needs the certificate pass) and
+// uploads the tables' arguments when the weights are bound.
+atg_status batch_set_split(atg_batch* b, int32_t split) {
+    b->opts.split_method = split;
+    for (BuildArgs& a : b->args) a.want_cert = split == ATG_SPLIT_LEARNED;
+    if (b->weights_set) {
+        HIPCHK(hipStreamSynchronize(b->last_stream));
+        HIPCHK(hipMemcpy(b->d_args, b->args.data(), sizeof(BuildArgs) * b->ntables, hipMemcpyHostToDevice));
+    }
+    return ATG_OK;
 }
 
 }  // namespace
@@ -424,66 +555,14 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         a.p_req = z.p_req;
         a.pack_cap = pack_cap_items(n, z.p_req);
         b->pack_cap_max = std::max(b->pack_cap_max, a.pack_cap);
-        b->G_max = std::max(b->G_max, z.G);
         a.splits = b->d_splits + osp[t];
         a.learned_segments = learned_knots(o.learned_segments, z.p_req);
     }
-    // Block->table maps of the phases (blocks per table as the single-table
-    // launchers use them).
-    auto blocks_of = [&](int ph, int64_t t) -> int64_t {
-        const Sizes& z = sz[t];
-        const int64_t n = offsets[t + 1] - offsets[t];
-        const int64_t K = std::min<int64_t>(learned_knots(o.learned_segments, z.p_req), z.p_req);
-        switch (ph) {
-            case MAP_SUM: return (z.G + 3) / 4;
-            case MAP_LOCAL: return z.G;
-            case MAP_PLACE: return (z.G + 3) / 4;
-            case MAP_TREE: return z.G > 1 ? (z.G + 511) / 512 : 0;  // one section: no tree
-            case MAP_DEG: return degenerate_blocks(n);
-            case MAP_SPLIT: return std::max<int64_t>(1, (z.p_req - 1 + 255) / 256);
-            case MAP_KNOTS: return (K + 1 + 255) / 256;
-            default: return (z.p_req + 63) / 64;  // MAP_PACK
-        }
-    };
-    std::vector<int64_t> base((size_t)MAP_COUNT * ntables);
-    int64_t totblocks[MAP_COUNT] = {};
-    for (int ph = 0; ph < MAP_COUNT; ++ph)
-        for (int64_t t = 0; t < ntables; ++t) {
-            base[(size_t)ph * ntables + t] = totblocks[ph];
-            totblocks[ph] += blocks_of(ph, t);
-        }
-    int64_t alltab = 0;
-    for (int ph = 0; ph < MAP_COUNT; ++ph) {
-        if (totblocks[ph] > 0x7FFFFFFF)
-            return cleanup(fail(ATG_EINVAL, "atg_batch_create: %lld blocks in one phase", (long long)totblocks[ph]));
-        alltab += totblocks[ph];
-    }
-    std::vector<int32_t> tab((size_t)alltab);
     {
-        int64_t pos = 0;
-        for (int ph = 0; ph < MAP_COUNT; ++ph)
-            for (int64_t t = 0; t < ntables; ++t) {
-                const int64_t nb = blocks_of(ph, t);
-                for (int64_t q = 0; q < nb; ++q) tab[(size_t)(pos + q)] = (int32_t)t;
-                pos += nb;
-            }
+        const atg_status rc = make_groups(b, o.batch_group_items);
+        if (rc != ATG_OK) return cleanup(rc);
     }
-    if (dalloc(&b->d_maptab, alltab) != hipSuccess || dalloc(&b->d_mapbase, base.size()) != hipSuccess) {
-        (void)hipGetLastError();
-        return cleanup(fail(ATG_ENOMEM, "atg_batch_create: block maps"));
-    }
-    {
-        int64_t pos = 0;
-        for (int ph = 0; ph < MAP_COUNT; ++ph) {
-            b->maps[ph].tab = b->d_maptab + pos;
-            b->maps[ph].base = b->d_mapbase + (size_t)ph * ntables;
-            b->maps[ph].blocks = totblocks[ph];
-            pos += totblocks[ph];
-        }
-    }
-    if (hipMemcpy(b->d_maptab, tab.data(), sizeof(int32_t) * alltab, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->d_mapbase, base.data(), sizeof(int64_t) * base.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->d_offsets, offsets, sizeof(int64_t) * (ntables + 1), hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpy(b->d_offsets, offsets, sizeof(int64_t) * (ntables + 1), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(b->d_state, 0, sizeof(BatchState) * ntables) != hipSuccess ||
         hipMemset(b->d_ticks, 0, sizeof(uint64_t) * 2 * ntables) != hipSuccess)
         return cleanup(fail(ATG_EDEVICE, "atg_batch_create: upload failed"));
@@ -531,6 +610,7 @@ atg_status atg_batch_build(atg_batch* b, void* stream) {
     if (!b) return fail(ATG_EINVAL, "atg_batch_build: NULL batch");
     if (!b->weights_set) return fail(ATG_ESTATE, "atg_batch_build: weights not set");
     hipStream_t s = (hipStream_t)stream;
+    b->last_stream = s;
     if (b->exec) HIPCHK(hipGraphLaunch(b->exec, s));
     else HIPCHK(b->dtype == ATG_F32 ? launch_batch<float>(b, s) : launch_batch<double>(b, s));
     b->built = true;
@@ -581,14 +661,20 @@ atg_status atg_batch_sync(atg_batch* b, void* stream) {
 atg_status atg_batch_get_tables(atg_batch* b, double* weights, int32_t* aliases, double* W_per_table) {
     if (!b) return fail(ATG_EINVAL, "atg_batch_get_tables: NULL batch");
     if (!b->built) return fail(ATG_ESTATE, "atg_batch_get_tables: not built");
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipStreamSynchronize(b->last_stream));
     const size_t n = (size_t)b->total;
     if (weights || aliases) {
-        std::vector<Row> rows(n);
-        HIPCHK(hipMemcpy(rows.data(), b->d_rows, n * sizeof(Row), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < n; ++i) {
-            if (weights) weights[i] = rows[i].weight;
-            if (aliases) aliases[i] = rows[i].alias;
+        // In chunks of 4 Mi rows (64 MiB): the rows of a large batch do not
+        // need a host copy of their own.
+        const size_t chunk = (size_t)1 << 22;
+        std::vector<Row> rows(std::min(n, chunk));
+        for (size_t c0 = 0; c0 < n; c0 += chunk) {
+            const size_t m = std::min(chunk, n - c0);
+            HIPCHK(hipMemcpy(rows.data(), b->d_rows + c0, m * sizeof(Row), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < m; ++i) {
+                if (weights) weights[c0 + i] = rows[i].weight;
+                if (aliases) aliases[c0 + i] = rows[i].alias;
+            }
         }
     }
     if (W_per_table) {
@@ -596,6 +682,107 @@ atg_status atg_batch_get_tables(atg_batch* b, double* weights, int32_t* aliases,
         HIPCHK(hipMemcpy(st.data(), b->d_state, sizeof(BatchState) * b->ntables, hipMemcpyDeviceToHost));
         for (int64_t t = 0; t < b->ntables; ++t) W_per_table[t] = st[t].s.W;
     }
+    return ATG_OK;
+}
+
+atg_status atg_batch_get_table(atg_batch* b, int64_t t, double* weights, int32_t* aliases, double* W) {
+    if (!b) return fail(ATG_EINVAL, "atg_batch_get_table: NULL batch");
+    if (t < 0 || t >= b->ntables) return fail(ATG_EINVAL, "atg_batch_get_table: table %lld of %lld", (long long)t, (long long)b->ntables);
+    if (!b->built) return fail(ATG_ESTATE, "atg_batch_get_table: not built");
+    HIPCHK(hipStreamSynchronize(b->last_stream));
+    const int64_t off = b->offsets[t], n = b->offsets[t + 1] - off;
+    if (weights || aliases) {
+        std::vector<Row> rows((size_t)n);
+        HIPCHK(hipMemcpy(rows.data(), b->d_rows + off, (size_t)n * sizeof(Row), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) {
+            if (weights) weights[i] = rows[i].weight;
+            if (aliases) aliases[i] = rows[i].alias;
+        }
+    }
+    if (W) {
+        BatchState st;
+        HIPCHK(hipMemcpy(&st, b->d_state + t, sizeof(BatchState), hipMemcpyDeviceToHost));
+        *W = st.s.W;
+    }
+    return ATG_OK;
+}
+
+atg_status atg_batch_groups(atg_batch* b, int32_t* group_of_table, int64_t* ngroups) {
+    if (!b) return fail(ATG_EINVAL, "atg_batch_groups: NULL batch");
+    if (ngroups) *ngroups = (int64_t)b->groups.size();
+    if (group_of_table) std::copy(b->group_of.begin(), b->group_of.end(), group_of_table);
+    return ATG_OK;
+}
+
+// Times the table-neutral knobs of a batch in place: the split method
+// (bisection or the learned index) and the launch groups (one group, or
+// groups of total/2^k items down to about 2^26 items).  Each candidate is the
+// median makespan of `reps` builds after a warm-up build; among the
+// candidates whose makespan is within (1 + makespan_slack) of the fastest, the
+// one with the lowest 99th-percentile table latency wins.  The batch keeps the
+// winning configuration (any captured graph is dropped).
+atg_status atg_batch_autotune(atg_batch* b, int32_t reps, double makespan_slack, atg_build_opts* chosen,
+                              double* makespan_ms, double* p99_latency_ms) {
+    if (!b) return fail(ATG_EINVAL, "atg_batch_autotune: NULL batch");
+    if (!b->weights_set) return fail(ATG_ESTATE, "atg_batch_autotune: weights not set");
+    if (reps < 1) reps = 3;
+    if (!(makespan_slack >= 0.0)) makespan_slack = 0.0;
+    drop_graph(b);
+    std::vector<int64_t> gsizes = {0};
+    for (int64_t gi = b->total / 2; gi >= ((int64_t)1 << 26) && (int64_t)gsizes.size() < 8; gi /= 2)
+        gsizes.push_back(gi);
+    struct Res { int32_t split; int64_t group_items; double ms, p99; };
+    std::vector<Res> res;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    atg_status rc = ATG_OK;
+    std::vector<double> lat((size_t)b->ntables);
+    for (int32_t split : {ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED}) {
+        if (rc != ATG_OK) break;
+        rc = batch_set_split(b, split);
+        for (int64_t gi : gsizes) {
+            if (rc != ATG_OK) break;
+            rc = make_groups(b, gi);
+            if (rc == ATG_OK) rc = atg_batch_build(b, nullptr);
+            if (rc == ATG_OK) rc = atg_batch_sync(b, nullptr);
+            std::vector<float> ts;
+            std::vector<double> p99s;
+            for (int r = 0; r < reps && rc == ATG_OK; ++r) {
+                (void)hipEventRecord(e0, nullptr);
+                rc = atg_batch_build(b, nullptr);
+                (void)hipEventRecord(e1, nullptr);
+                if (rc == ATG_OK) rc = atg_batch_sync(b, nullptr);
+                float ms = 0.f;
+                if (rc == ATG_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(ATG_EDEVICE, "events");
+                if (rc == ATG_OK) rc = atg_batch_latencies(b, lat.data());
+                if (rc != ATG_OK) break;
+                ts.push_back(ms);
+                std::vector<double> l = lat;
+                const size_t k = (size_t)std::min<double>((double)(l.size() - 1), std::ceil(0.99 * (double)l.size()) - 1);
+                std::nth_element(l.begin(), l.begin() + (long)k, l.end());
+                p99s.push_back(l[k] * 1e-6);
+            }
+            if (rc != ATG_OK) break;
+            std::sort(ts.begin(), ts.end());
+            std::sort(p99s.begin(), p99s.end());
+            res.push_back(Res{split, gi, ts[ts.size() / 2], p99s[p99s.size() / 2]});
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != ATG_OK) return rc;
+    double fastest = 1e300;
+    for (const Res& r : res) fastest = std::min(fastest, r.ms);
+    const Res* win = nullptr;
+    for (const Res& r : res)
+        if (r.ms <= fastest * (1.0 + makespan_slack) && (!win || r.p99 < win->p99)) win = &r;
+    rc = batch_set_split(b, win->split);
+    if (rc == ATG_OK) rc = make_groups(b, win->group_items);
+    if (rc != ATG_OK) return rc;
+    if (chosen) *chosen = b->opts;
+    if (makespan_ms) *makespan_ms = win->ms;
+    if (p99_latency_ms) *p99_latency_ms = win->p99;
     return ATG_OK;
 }
 
@@ -620,7 +807,7 @@ atg_status atg_batch_sample(atg_batch* b, uint64_t per_table, uint64_t seed, int
 
 atg_status atg_batch_latencies(atg_batch* b, double* ns_per_table) {
     if (!b || !ns_per_table) return fail(ATG_EINVAL, "atg_batch_latencies: NULL argument");
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipStreamSynchronize(b->last_stream));
     std::vector<uint64_t> tk((size_t)(2 * b->ntables));
     HIPCHK(hipMemcpy(tk.data(), b->d_ticks, sizeof(uint64_t) * 2 * b->ntables, hipMemcpyDeviceToHost));
     const double ns_per_tick = 1e6 / b->wall_khz;

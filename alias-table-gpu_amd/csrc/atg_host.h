@@ -19,9 +19,12 @@ int device_cus();
 atg_status state_status(const DevState& st);
 // Sets the chain count and requested splits of a created table in place.
 atg_status table_set_shape(atg_table* t, int32_t chains, int64_t num_splits);
+// Sets the split method, knot count and split/pack interleave of a created
+// table (knobs that leave the table's rows unchanged).
+atg_status table_set_method(atg_table* t, int32_t split_method, int32_t learned_segments, int32_t interleave);
 
 constexpr int64_t kMaxSectionLen = 2048;
-constexpr int kInterleaveChunks = 8;  // split/pack interleave chunks  // items of one section staged in registers + LDS (PSA_MAX_LEN)
+constexpr int kInterleaveChunks = 8;  // split/pack interleave chunks
 constexpr int64_t kMaxN = 0x7FFFFFF0LL;
 
 template <typename T>

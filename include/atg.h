@@ -65,6 +65,11 @@ typedef struct {
     int32_t interleave;      /* 1: atg_build runs split and pack interleaved on two streams
                                 (performSplitAndPackInterleaved, AliasTableSplitGpu.cu:220-262);
                                 not with record_phases.  Same table either way. */
+    int64_t batch_group_items; /* batch only: the tables are dealt into groups of about this many
+                                items (largest table first, each to the lightest group) and the
+                                phases run group after group, so a table's build latency is its
+                                group's, not the whole batch's; <= 0: one group.  Same tables
+                                either way. */
 } atg_build_opts;
 
 typedef struct {
@@ -245,6 +250,14 @@ atg_status atg_batch_build(atg_batch* b, void* stream);
 atg_status atg_batch_sync(atg_batch* b, void* stream);
 /* Rows (AoS, 16 B) of all tables, concatenated; per-table W. */
 atg_status atg_batch_get_tables(atg_batch* b, double* weights, int32_t* aliases, double* W_per_table);
+/* Rows and W of table t alone (host buffers of offsets[t+1] - offsets[t]
+ * entries; any pointer may be NULL): AliasTable::aliasTable of one table
+ * (AliasTable.cuh:23) without copying the whole batch. */
+atg_status atg_batch_get_table(atg_batch* b, int64_t t, double* weights, int32_t* aliases, double* W);
+/* The number of groups the tables were dealt into (batch_group_items) and,
+ * when group_of_table is not NULL, the group of every table (ntables entries).
+ * Batch bookkeeping; no counterpart in the reference. */
+atg_status atg_batch_groups(atg_batch* b, int32_t* group_of_table, int64_t* ngroups);
 /* enable = 1: captures the build's launch sequence (kernels only) into a
  * hipGraph that atg_batch_build replays from then on; 0: back to direct
  * launches.  The graph stays valid across atg_batch_set_weights.  Replaces
@@ -260,15 +273,34 @@ atg_status atg_batch_latencies(atg_batch* b, double* ns_per_table);
 
 /* ---- autotuner ----------------------------------------------------------- */
 
-/* Chooses section_size, chains and num_splits for these weights on this
- * device: builds them with each candidate of a small grid (median of `reps`
- * timed builds after a warm-up) and writes the fastest options to *best (the
- * other fields come from *base, or the defaults).  *best_ms: the median build
- * time, negative when the choice came from the ATG_TUNE_CACHE file.
- * No counterpart in the reference (its shape is compile-time,
- * PsaPlusNormal.cu:3, AliasTableSplitGpu.cu:58-71,356). */
+/* Chooses the knobs that leave the table unchanged for these weights and the
+ * build shape in *base (section_size, chains, num_splits, PSA+; NULL: the
+ * defaults): the split method, the learned index's knot count and the
+ * split/pack interleave.  Times each candidate (median of `reps` builds after
+ * a warm-up) and writes *base with the fastest knobs to *best, so every
+ * candidate builds the same table.  *best_ms: the median build time, negative
+ * when the choice came from the ATG_TUNE_CACHE file.  No counterpart in the
+ * reference (its split and pack methods are compile-time choices,
+ * AliasTableSplitGpu.cuh:60-100). */
 atg_status atg_autotune(const void* weights, atg_mem mem, int64_t n, atg_dtype dtype, atg_layout layout,
                         const atg_build_opts* base, int32_t reps, atg_build_opts* best, double* best_ms);
+/* Searches the build shape: section_size x chains x num_splits over a small
+ * grid, the fastest median build wins.  The shape changes the table (W's
+ * association, the leftovers and every row), so the result is meant as an
+ * explicit input to atg_create, chosen once and recorded (atg_save_table
+ * stores it); builds with one shape are deterministic.  No counterpart in the
+ * reference (its shape is compile-time, PsaPlusNormal.cu:3,
+ * AliasTableSplitGpu.cu:58-71,356). */
+atg_status atg_autotune_shape(const void* weights, atg_mem mem, int64_t n, atg_dtype dtype, atg_layout layout,
+                              const atg_build_opts* base, int32_t reps, atg_build_opts* best, double* best_ms);
+/* Tunes a batch in place over the knobs that leave its tables unchanged: the
+ * split method and the launch groups (batch_group_items).  Among the
+ * candidates whose median makespan is within (1 + makespan_slack) of the
+ * fastest, the one with the lowest 99th-percentile table latency wins and
+ * stays configured; *chosen receives the batch's options.  Weights must be
+ * set.  No counterpart in the reference. */
+atg_status atg_batch_autotune(atg_batch* b, int32_t reps, double makespan_slack, atg_build_opts* chosen,
+                              double* makespan_ms, double* p99_latency_ms);
 
 /* ---- misc --------------------------------------------------------------- */
 const char* atg_last_error(void);

@@ -784,3 +784,144 @@ extern "C" void oracle_ref_stack_build(const double* wt, int64_t N, double* W_ou
     }
     *W_out = W;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Batches of independent tables (test infrastructure for the engine's
+ * atg_batch_*, which has no counterpart in the reference).  Table t owns
+ * w[offsets[t], offsets[t+1]) and is built exactly as oracle_build builds it
+ * alone, with the per-table options the batch derives: section parameter
+ * o->section_size, or min(1000, n/10) when it is <= 0 (AliasTableSplitGpu.cu:356);
+ * splits o->num_splits, one per -num_splits items when it is negative.  One
+ * table per thread. */
+static void batch_table_opts(const oracle_opts* o, int64_t n, oracle_opts* t) {
+    *t = *o;
+    t->threads = 1;
+    if (t->section_size <= 0) t->section_size = std::max<int64_t>(1, std::min<int64_t>(1000, n / 10));
+    if (t->num_splits < 0) t->num_splits = std::max<int64_t>(2, n / -o->num_splits);
+}
+
+static void batch_weights(const void* w, int32_t w_f32, int64_t off, int64_t n, std::vector<double>& out) {
+    out.resize((size_t)n);
+    if (w_f32) {
+        const float* f = (const float*)w + off;
+        for (int64_t i = 0; i < n; ++i) out[(size_t)i] = (double)f[i];
+    } else {
+        std::memcpy(out.data(), (const double*)w + off, sizeof(double) * (size_t)n);
+    }
+}
+
+extern "C" int oracle_build_batch(const void* w, int32_t w_f32, const int64_t* offsets, int64_t ntables,
+                                  const int64_t* which, int64_t nwhich, const oracle_opts* o, double* row_w,
+                                  int32_t* row_alias, double* W_out, int32_t threads) {
+    const int64_t cnt = which ? nwhich : ntables;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1) reduction(| : bad)
+    for (int64_t q = 0; q < cnt; ++q) {
+        const int64_t t = which ? which[q] : q;
+        const int64_t off = offsets[t], n = offsets[t + 1] - off;
+        std::vector<double> x;
+        batch_weights(w, w_f32, off, n, x);
+        oracle_opts to;
+        batch_table_opts(o, n, &to);
+        oracle_stats st;
+        if (oracle_build(x.data(), n, &to, row_w + off, row_alias + off, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, nullptr, nullptr, nullptr, &st) != 0) {
+            bad |= 1;
+            continue;
+        }
+        if (W_out) W_out[t] = st.W;
+    }
+    return bad;
+}
+
+/* Per table: the canonical W (section parameter as oracle_build_batch derives
+ * it) and the reference's table check of AliasTable.cu:24-65 in double
+ * precision, as the reference computes it: implied mass min(row, W/N) plus
+ * W/N - row for every row aliased to the item, against the weight, within
+ * max(n 1e-8, 1e-3), and row weights within [0, W/N (1 + that bound)].
+ * W_table is the W the table was built with. */
+extern "C" void oracle_check_batch(const void* w, int32_t w_f32, const int64_t* offsets, int64_t ntables,
+                                   const oracle_opts* o, const double* row_w, const int32_t* row_alias,
+                                   const double* W_table, double* W_canonical, int32_t* ok, double* max_abs,
+                                   int32_t threads) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+    for (int64_t t = 0; t < ntables; ++t) {
+        const int64_t off = offsets[t], n = offsets[t + 1] - off;
+        std::vector<double> x;
+        batch_weights(w, w_f32, off, n, x);
+        oracle_opts to;
+        batch_table_opts(o, n, &to);
+        int64_t G, Ls;
+        oracle_section_layout(n, to.section_size, &G, &Ls);
+        std::vector<double> sums((size_t)G);
+        for (int64_t s = 0; s < G; ++s) {
+            const int64_t a = std::min(n, s * Ls), b = std::min(n, (s + 1) * Ls);
+            sums[(size_t)s] = oracle_section_sum(x.data() + a, b - a);
+        }
+        W_canonical[t] = oracle_tree64_reduce(sums.data(), G);
+        const double W_N = W_table[t] / (double)n;
+        std::vector<double> m((size_t)n, 0.0);
+        const double* rw = row_w + off;
+        const int32_t* ra = row_alias + off;
+        int good = 1;
+        const double maxError = std::max((double)n * 1e-8, 1e-3);
+        for (int64_t i = 0; i < n; ++i) {
+            if (rw[i] < W_N) {
+                m[(size_t)i] += rw[i];
+                /* an alias that is not an item keeps the mass on the row, as
+                 * the samplers do (a row a rounding step below W/N may carry
+                 * the heavy sentinel) */
+                if (ra[i] >= 0 && ra[i] < n) m[(size_t)ra[i]] += W_N - rw[i];
+                else m[(size_t)i] += W_N - rw[i];
+            } else {
+                m[(size_t)i] += W_N;
+            }
+            if (!(rw[i] >= 0.0) || rw[i] > W_N * (1.0 + maxError)) good = 0;
+        }
+        double mabs = 0.0;
+        for (int64_t i = 0; i < n; ++i) mabs = std::max(mabs, std::fabs(m[(size_t)i] - x[(size_t)i]));
+        if (!(mabs <= maxError)) good = 0;
+        ok[t] = good;
+        max_abs[t] = mabs;
+    }
+}
+
+/* Sectioned sampling restricted to some sections: the samples that
+ * oracle_sample_sectioned writes for sections sections[0..nsel) (counts from
+ * the whole halving tree), concatenated in the given order; first[q] is the
+ * output position of section sections[q]'s first sample in the full run and
+ * cnt[q] its count. */
+extern "C" void oracle_sample_sectioned_subset(const double* row_w, const int32_t* row_alias, int64_t n,
+                                              double W_N, int64_t SS, uint64_t total, uint64_t seed,
+                                              const int64_t* sections, int64_t nsel, int32_t* out,
+                                              uint64_t* first, uint64_t* cnt) {
+    const int64_t nsec = (n + SS - 1) / SS;
+    std::vector<uint64_t> counts((size_t)nsec);
+    oracle_section_counts(n, SS, total, seed, counts.data());
+    std::vector<uint64_t> off((size_t)nsec + 1, 0);
+    for (int64_t s = 0; s < nsec; ++s) off[(size_t)s + 1] = off[(size_t)s] + counts[(size_t)s];
+    uint64_t pos = 0;
+    for (int64_t q = 0; q < nsel; ++q) {
+        const int64_t s = sections[q];
+        first[q] = off[(size_t)s];
+        cnt[q] = counts[(size_t)s];
+        const int64_t start = s * SS, len = std::min(n, (s + 1) * SS) - start;
+        for (uint64_t t = 0; t < counts[(size_t)s]; ++t) {
+            const uint64_t g = off[(size_t)s] + t;
+            uint32_t x[4];
+            philox_u64(g, ORACLE_PURPOSE_SECTIONED, 0, seed, x);
+            const int64_t row = start + (int64_t)mulhi64(word64(x), (uint64_t)len);
+            out[pos++] = decide(row_w[row], row_alias[row], row, coin53(x), W_N, n);
+        }
+    }
+}
+
+/* Threads of the oracle's OpenMP loops (the CPU baselines time 1 thread and
+ * all host cores). */
+extern "C" void oracle_set_threads(int32_t threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+}

@@ -133,6 +133,21 @@ void oracle_ref_sweep_build(double* w_n2, int64_t n, double* W_out, double* row_
 void oracle_ref_stack_build(const double* w, int64_t n, double* W_out, double* row_w,
                             int32_t* row_alias);
 
+/* Batches of independent tables (the engine's atg_batch_*): see atg_oracle.cpp. */
+int oracle_build_batch(const void* w, int32_t w_f32, const int64_t* offsets, int64_t ntables,
+                       const int64_t* which /* nullable: all */, int64_t nwhich, const oracle_opts* o,
+                       double* row_w, int32_t* row_alias, double* W_out /* nullable, ntables */,
+                       int32_t threads);
+void oracle_check_batch(const void* w, int32_t w_f32, const int64_t* offsets, int64_t ntables,
+                        const oracle_opts* o, const double* row_w, const int32_t* row_alias,
+                        const double* W_table, double* W_canonical, int32_t* ok, double* max_abs,
+                        int32_t threads);
+void oracle_sample_sectioned_subset(const double* row_w, const int32_t* row_alias, int64_t n, double W_N,
+                                    int64_t section_rows, uint64_t total, uint64_t seed,
+                                    const int64_t* sections, int64_t nsel, int32_t* out, uint64_t* first,
+                                    uint64_t* cnt);
+void oracle_set_threads(int32_t threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,14 @@ L.oracle_ref_sweep_build.argtypes = [P, C.c_int64, P, P, P]
 L.oracle_ref_stack_build.argtypes = [P, C.c_int64, P, P, P]
 
 
+L.oracle_build_batch.argtypes = [P, C.c_int32, P, C.c_int64, P, C.c_int64, P, P, P, P, C.c_int32]
+L.oracle_build_batch.restype = C.c_int
+L.oracle_check_batch.argtypes = [P, C.c_int32, P, C.c_int64, P, P, P, P, P, P, P, C.c_int32]
+L.oracle_sample_sectioned_subset.argtypes = [P, P, C.c_int64, C.c_double, C.c_int64, C.c_uint64, C.c_uint64, P,
+                                             C.c_int64, P, P, P]
+L.oracle_set_threads.argtypes = [C.c_int32]
+
+
 def ptr(a):
     return None if a is None else a.ctypes.data_as(P)
 
@@ -225,3 +233,65 @@ def ref_stack_build(w):
     W = C.c_double()
     L.oracle_ref_stack_build(ptr(w), n, C.byref(W), ptr(rw), ptr(ra))
     return W.value, rw, ra
+
+
+def _batch_opts(psa_plus, chains, section_size, num_splits):
+    return Opts(1 if psa_plus else 0, int(chains), int(section_size), int(num_splits), 1)
+
+
+def build_batch(w, offsets, which=None, psa_plus=True, chains=16, section_size=0, num_splits=0, threads=16):
+    """Rows (concatenated; rows of tables not in `which` stay 0 / -1) and W of
+    the tables of a batch, each built as oracle.build would build it alone
+    with the options the engine's batch derives per table (section_size 0:
+    min(1000, n/10); num_splits < 0: one split per -num_splits items)."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    f32 = w.dtype == np.float32
+    w = np.ascontiguousarray(w)
+    total = int(offsets[-1])
+    rw = np.zeros(total)
+    ra = np.full(total, -1, np.int32)
+    W = np.zeros(len(offsets) - 1)
+    sel = None if which is None else np.ascontiguousarray(which, dtype=np.int64)
+    o = _batch_opts(psa_plus, chains, section_size, num_splits)
+    r = L.oracle_build_batch(ptr(w), 1 if f32 else 0, ptr(offsets), len(offsets) - 1, ptr(sel),
+                             0 if sel is None else len(sel), C.byref(o), ptr(rw), ptr(ra), ptr(W), int(threads))
+    if r != 0:
+        raise ValueError("oracle_build_batch rejected a table")
+    return rw, ra, W
+
+
+def check_batch(w, offsets, row_w, row_alias, W_table, psa_plus=True, chains=16, section_size=0, num_splits=0,
+                threads=16):
+    """Per table: the canonical W and the reference's table check
+    (AliasTable.cu:24-65, double precision): (W_canonical, ok, max_abs)."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    f32 = w.dtype == np.float32
+    nt = len(offsets) - 1
+    Wc = np.zeros(nt)
+    ok = np.zeros(nt, np.int32)
+    mabs = np.zeros(nt)
+    o = _batch_opts(psa_plus, chains, section_size, num_splits)
+    L.oracle_check_batch(ptr(np.ascontiguousarray(w)), 1 if f32 else 0, ptr(offsets), nt, C.byref(o),
+                         ptr(np.ascontiguousarray(row_w)), ptr(np.ascontiguousarray(row_alias, dtype=np.int32)),
+                         ptr(np.ascontiguousarray(W_table, dtype=np.float64)), ptr(Wc), ptr(ok), ptr(mabs),
+                         int(threads))
+    return Wc, ok.astype(bool), mabs
+
+
+def sample_sectioned_subset(row_w, row_alias, W_N, section_rows, total, seed, sections):
+    """Samples of the given sections of a sectioned run (concatenated), the
+    first output position and the count of every given section."""
+    sections = np.ascontiguousarray(sections, dtype=np.int64)
+    n = len(row_w)
+    counts = section_counts(n, section_rows, total, seed)
+    out = np.empty(int(counts[sections].sum()), np.int32)
+    first = np.empty(len(sections), np.uint64)
+    cnt = np.empty(len(sections), np.uint64)
+    L.oracle_sample_sectioned_subset(ptr(np.ascontiguousarray(row_w)), ptr(np.ascontiguousarray(row_alias)), n,
+                                     float(W_N), int(section_rows), int(total), int(seed), ptr(sections),
+                                     len(sections), ptr(out), ptr(first), ptr(cnt))
+    return out, first, cnt
+
+
+def set_threads(threads):
+    L.oracle_set_threads(int(threads))

@@ -180,3 +180,53 @@ def test_batch_graph_replays_match_direct_launches(gpu, oracle):
     b.sync()
     got = b.tables()
     assert np.array_equal(bits(got[0]), bits(ref[0])) and np.array_equal(got[1], ref[1])
+
+
+def test_batch_groups_identical_tables(gpu, oracle):
+    """Tables dealt into launch groups (largest first, each to the lightest
+    group) build the same rows as one group, directly and from a replayed
+    graph; every group holds tables; a table's latency is its group's."""
+    sizes = [int(x) for x in np.random.default_rng(3).integers(16, 70000, size=200)]
+    offsets, w = make_batch(sizes, "exponential", seed=21)
+    kw = dict(section_size=2048, chains=32, num_splits=-38)
+    one = atg.Batch(offsets, **kw)
+    one.set_weights(w)
+    one.build()
+    one.sync()
+    rw1, ra1, W1 = one.tables()
+    for group_items, graph in ((offsets[-1] // 5, False), (offsets[-1] // 3, True)):
+        b = atg.Batch(offsets, group_items=group_items, **kw)
+        g, ng = b.groups()
+        assert ng == -(-int(offsets[-1]) // group_items) and set(g.tolist()) == set(range(ng))
+        b.set_weights(w)
+        if graph:
+            b.use_graph(True)
+        for _ in range(2):
+            b.build()
+            b.sync()
+            rw, ra, W = b.tables()
+            assert np.array_equal(bits(rw), bits(rw1)) and np.array_equal(ra, ra1) and np.array_equal(bits(W), bits(W1))
+        for t in (0, 57, 199):
+            tw, ta, tW = b.table(t)
+            lo, hi = offsets[t], offsets[t + 1]
+            assert np.array_equal(bits(tw), bits(rw1[lo:hi])) and np.array_equal(ta, ra1[lo:hi]) and tW == W1[t]
+        assert np.all(b.latencies_ns() > 0)
+    ref_rw, ref_ra, ref_W = oracle.build_batch(w, offsets, **kw)
+    assert np.array_equal(bits(rw1), bits(ref_rw)) and np.array_equal(ra1, ref_ra) and np.array_equal(bits(W1), bits(ref_W))
+
+
+def test_batch_autotune_keeps_tables(gpu, oracle):
+    """The batch tuner varies the split method and the launch groups only:
+    the tables after tuning equal the oracle's."""
+    sizes = [int(x) for x in np.random.default_rng(4).integers(1000, 300000, size=300)]
+    offsets, w = make_batch(sizes, "pareto", seed=31)
+    kw = dict(section_size=2048, chains=32, num_splits=-38)
+    b = atg.Batch(offsets, **kw)
+    b.set_weights(w)
+    res = b.autotune(reps=1, makespan_slack=0.1)
+    assert res["split"] in ("bisect", "learned") and res["makespan_ms"] > 0 and res["p99_latency_ms"] > 0
+    b.build()
+    b.sync()
+    rw, ra, W = b.tables()
+    ref_rw, ref_ra, ref_W = oracle.build_batch(w, offsets, **kw)
+    assert np.array_equal(bits(rw), bits(ref_rw)) and np.array_equal(ra, ref_ra) and np.array_equal(bits(W), bits(ref_W))

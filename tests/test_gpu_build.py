@@ -161,21 +161,41 @@ def test_all_zero_weights_rejected(gpu):
         t.sync()
 
 
-def test_autotune_returns_valid_shape(gpu, oracle, tmp_path, monkeypatch):
-    """The tuner picks a shape from its grid; that shape builds the oracle's
-    table; a second call with the cache file returns the cached choice."""
+def test_autotune_shape_returns_valid_shape(gpu, oracle, tmp_path, monkeypatch):
+    """The shape search picks a shape from its grid; that shape builds the
+    oracle's table; a second call with the cache file returns the cached
+    choice."""
     monkeypatch.setenv("ATG_TUNE_CACHE", str(tmp_path / "tune.txt"))
     n = 400_000
     w = atg.generate_weights("pareto", n, seed=3, param=1.1)
-    shape, ms = atg.autotune(w, reps=1)
+    shape, ms = atg.autotune_shape(w, reps=1)
     assert ms > 0
     assert shape["section_size"] in (1024, 1536, 2048) and shape["chains"] in (12, 16, 24, 32)
     assert 2 <= shape["num_splits"] <= n // 10
     t = gpu_build(w, chains=shape["chains"], section_size=shape["section_size"], num_splits=shape["num_splits"])
     ref = oracle.build(w, chains=shape["chains"], section_size=shape["section_size"], num_splits=shape["num_splits"])
     assert_same_build(t, ref, n)
-    shape2, ms2 = atg.autotune(w, reps=1)
+    shape2, ms2 = atg.autotune_shape(w, reps=1)
     assert ms2 < 0 and shape2 == shape
+
+
+@pytest.mark.parametrize("dist", ["pareto", "heavy_mix"])
+def test_autotune_leaves_table_unchanged(gpu, oracle, monkeypatch, dist):
+    """The run-time tuner varies only knobs that keep the rows: two uncached
+    calls at one explicit shape build byte-identical tables, equal to the
+    oracle's table for that shape, whatever knobs win."""
+    monkeypatch.delenv("ATG_TUNE_CACHE", raising=False)
+    n = 600_000
+    w = atg.generate_weights(dist, n, seed=8, param=1.1 if dist == "pareto" else 0.9)
+    shape = dict(chains=24, section_size=2048, num_splits=n // 64)
+    ref = oracle.build(w, want_lists=False, **shape)
+    for rep in range(2):
+        knobs, ms = atg.autotune(w, reps=1, **shape)
+        assert ms > 0 and knobs["split"] in ("bisect", "learned")
+        t = gpu_build(w, split=knobs["split"], learned_segments=knobs["learned_segments"],
+                      interleave=knobs["interleave"], **shape)
+        tw, ta = t.table()
+        assert np.array_equal(bits(tw), bits(ref["row_w"])) and np.array_equal(ta, ref["row_alias"]), (rep, knobs)
 
 
 def test_split_ratio_mode(gpu, oracle):
@@ -212,3 +232,22 @@ def test_interleaved_split_pack_identical(gpu, oracle, n, dist, split, layout, p
         tw, ta = t.table()
         assert np.array_equal(bits(tw), bits(ref["row_w"])) and np.array_equal(ta, ref["row_alias"]), rep
     assert t.stats()["p"] == ref["stats"]["p"]
+
+
+@pytest.mark.parametrize("corrupt", ["row_weight", "alias"])
+def test_post_build_rejects_corrupted_table(gpu, oracle, corrupt):
+    """atg_post_build (postBuild + verifyTableCorrectness, AliasTable.cu:24-65)
+    passes the built table and rejects the same rows with one row weight or
+    one alias changed, loaded with atg_set_table."""
+    import test_oracle as pins
+    n = 20000
+    w = atg.generate_weights("sine", n)
+    t = gpu_build(w, num_splits=1000)
+    rel, ok = t.post_build()
+    assert ok and rel < 1e-9
+    ref = oracle.build(w, num_splits=1000, want_lists=False)
+    fn = pins.corrupt_row_weight if corrupt == "row_weight" else pins.corrupt_alias
+    rw, ra = fn(ref)
+    t.set_table(rw, ra, t.stats()["W"])
+    rel, ok = t.post_build()
+    assert not ok and rel > 1e-3

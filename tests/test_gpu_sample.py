@@ -130,3 +130,23 @@ def test_rejection_f32_weights(gpu, oracle, n, dist):
     t.rejection_sample(count, seed=21, out=out)
     exp, _ = oracle.sample_rejection(A, w64, W, 21, count)
     np.testing.assert_array_equal(out.cpu().numpy(), exp)
+
+
+def test_basic_decision_boundaries(gpu, oracle):
+    """Hand-built rows on the boundaries of the rule of SamplerGpuBasic.cu:28-45
+    (a row weight equal to coin * W/N, one ulp above it, equal to W/N, an
+    always-alias weight whose alias is the heavy sentinel N, and a zero weight),
+    loaded with atg_set_table: the GPU returns the items the rule fixes and
+    the oracle's items."""
+    import test_oracle as pins
+    torch = gpu
+    n, seed, count = 1 << 16, 77, 200
+    rw, ra, expected = pins.decision_boundary_table(oracle, n, seed, count)
+    t = atg.Table(n)
+    t.set_table(rw, ra, float(n))  # W/N = 1
+    out = torch.empty(count, dtype=torch.int32, device="cuda")
+    t.sample(count, seed=seed, kind="basic", out=out)
+    got = out.cpu().numpy()
+    for k, exp in expected.items():
+        assert got[k] == exp, (k, k % 5, got[k], exp)
+    np.testing.assert_array_equal(got, oracle.sample_basic(rw, ra, 1.0, seed, count))

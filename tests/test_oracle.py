@@ -74,35 +74,70 @@ def test_sum_exact_on_integers(oracle, n):
         assert W == float(int(w.astype(np.int64).sum()))
 
 
-def test_sum_tree_shape(oracle):
-    # 64 items in one section: the lane sums are the items themselves and the
-    # total is the pairwise tree ((x0+x1)+(x2+x3))+... .  Built with numpy.
-    x = np.random.default_rng(3).random(64) * np.logspace(0, 15, 64)
-    t = x.copy()
-    while len(t) > 1:
-        t = t[0::2] + t[1::2]
-    W, _ = oracle.weight_sum(x, 64)
-    assert W == t[0]
+def _pairwise64(v):
+    """Pairwise trees of consecutive groups of 64 (zero-padded): t[m] = t[m] +
+    t[m + width] for width 1, 2, 4, ..., 32; returns t[0] of every group."""
+    v = np.concatenate([v, np.zeros((-len(v)) % 64)])
+    t = v.reshape(-1, 64).copy()
+    width = 1
+    while width < 64:
+        t[:, 0::2 * width] = t[:, 0::2 * width] + t[:, width::2 * width]
+        width *= 2
+    return t[:, 0]
 
 
-def test_scan_closed_form(oracle):
-    # PrefixSum::unitTest (PrefixSum.cu:265-296): input i % 7, exclusive prefix.
-    v = (np.arange(1000) % 7).astype(np.float64)
-    ex, agg = oracle.local_scan(v)
-    assert np.array_equal(ex, np.concatenate([[0.0], np.cumsum(v)[:-1]]))
-    assert agg == v.sum()
+def _canonical_sum_numpy(w, section_param, reverse_lanes=False):
+    """The canonical W written out with numpy from its definition: sections
+    of the PSA+ layout; in a section lane l adds items l, l + 64, ... left to
+    right; the 64 lane sums meet in a pairwise tree; the section sums meet in
+    recursive pairwise trees of 64.  Returns W and the section sums.
+    reverse_lanes: each lane adds its items right to left instead (a changed
+    association, to show that the check below can see one)."""
+    n = len(w)
+    G = n // section_param + 1
+    Ls = n // G + 1
+    sums = np.empty(G)
+    for s in range(G):
+        x = w[min(n, s * Ls):min(n, (s + 1) * Ls)]
+        acc = np.zeros(64)
+        starts = list(range(0, len(x), 64))
+        for k in (starts[::-1] if reverse_lanes else starts):
+            chunk = x[k:k + 64]
+            acc[:len(chunk)] = acc[:len(chunk)] + chunk
+        sums[s] = _pairwise64(acc)[0]
+    top = sums.copy()
+    while len(top) > 1:
+        top = _pairwise64(top)
+    return top[0], sums
 
 
-def test_scan_total_is_pairwise_chunk_tree(oracle):
-    x = np.random.default_rng(5).random(256) * np.logspace(0, 12, 256)
-    _, agg = oracle.local_scan(x)
-    carry = 0.0
-    for c in range(4):
-        t = x[64 * c:64 * (c + 1)].copy()
-        while len(t) > 1:
-            t = t[0::2] + t[1::2]
-        carry = carry + t[0]
-    assert agg == carry
+def _sequential_sum(w):
+    s = 0.0
+    for v in w.tolist():
+        s = s + v
+    return s
+
+
+@pytest.mark.parametrize("n,section_param", [(2583, 10**6), (64 * 64 * 5 + 17, 997), (300_007, 1000)])
+def test_sum_association_fixed(oracle, n, section_param):
+    # Values spread over 2^24 (no single term dominates a section, but the
+    # order of the additions shows in the low bits): the oracle's W and every
+    # section sum equal the definition written out independently with numpy,
+    # while a plain left-to-right sum (the reference's sumWeightsCpu,
+    # SamplingAlgorithm.cu:14-19) and lanes summed right to left give other
+    # bits, so a changed association fails this check.
+    rng = np.random.default_rng(n)
+    w = rng.random(n) * 2.0 ** rng.integers(0, 24, n)
+    W, sums = oracle.weight_sum(w, section_param)
+    W_np, sums_np = _canonical_sum_numpy(w, section_param)
+    assert np.array_equal(sums.view(np.uint64), sums_np.view(np.uint64))
+    assert W == W_np
+    assert W != _sequential_sum(w)
+    _, sums_rev = _canonical_sum_numpy(w, section_param, reverse_lanes=True)
+    assert np.any(sums_rev != sums)
+    G, Ls = oracle.section_layout(n, section_param)
+    if G == 1:
+        assert Ls > n  # one section holds every item
 
 
 # ---- construction ------------------------------------------------------------
@@ -141,6 +176,96 @@ def test_tables_pass_reference_check(oracle, dist, psa):
     if W_N > 1e-3:
         assert v["max_rel"] < 1e-7, v
     assert ref["stats"]["split_fallbacks"] == 0
+
+
+def corrupt_row_weight(ref):
+    """A copy of the table with one row weight moved by a quarter of W/N (a
+    row well below W/N whose alias is an item): mass moves between the row's
+    item and its alias."""
+    rw, ra = ref["row_w"].copy(), ref["row_alias"].copy()
+    W_N = ref["stats"]["W_N"]
+    i = int(np.nonzero((rw < 0.5 * W_N) & (ra >= 0) & (ra < len(rw)) & (ra != np.arange(len(rw))))[0][0])
+    rw[i] += 0.25 * W_N
+    return rw, ra
+
+
+def corrupt_alias(ref):
+    """A copy of the table with one alias pointed at another item (a row whose
+    alias carries at least W/N / 4 of mass)."""
+    rw, ra = ref["row_w"].copy(), ref["row_alias"].copy()
+    W_N = ref["stats"]["W_N"]
+    n = len(rw)
+    i = int(np.nonzero((rw < 0.75 * W_N) & (ra >= 0) & (ra < n))[0][0])
+    ra[i] = (ra[i] + 1) % n if (ra[i] + 1) % n != i else (ra[i] + 2) % n
+    return rw, ra
+
+
+@pytest.mark.parametrize("corrupt", [corrupt_row_weight, corrupt_alias])
+def test_verify_rejects_corrupted_tables(oracle, corrupt):
+    # The table check that every PSA+ pin relies on must fail on a wrong row:
+    # the reference criterion (AliasTable.cu:24-65) and the relative error.
+    n = 20000
+    w = atg.generate_weights("sine", n)
+    ref = oracle.build(w, num_splits=1000)
+    good = _verify(oracle, w, ref)
+    assert good["reference_ok"] and good["max_rel"] < 1e-9
+    rw, ra = corrupt(ref)
+    bad = oracle.verify(w, ref["stats"]["W"], ref["stats"]["W_N"], rw, ra)
+    assert not bad["reference_ok"], bad
+    assert bad["max_rel"] > 1e-3, bad
+
+
+PURPOSE_BASIC = 0x0000B001
+
+
+def basic_row_and_coin(oracle, seed, k, n):
+    """Row and coin of basic sample k: Philox4x32-10 (pinned above) at counter
+    (k, PURPOSE_BASIC), key = seed; row = floor(x64 n / 2^64) from words 0-1,
+    coin = the top 53 bits of words 2-3 as a double in [0, 1)."""
+    x = oracle.philox([k & 0xFFFFFFFF, k >> 32, PURPOSE_BASIC, 0], [seed & 0xFFFFFFFF, seed >> 32])
+    row = (((x[1] << 32) | x[0]) * n) >> 64
+    coin = (((x[3] << 32) | x[2]) >> 11) * 2.0 ** -53
+    return row, coin
+
+
+def decision_boundary_table(oracle, n, seed, count):
+    """A table of n rows with W/N = 1 built around the first `count` basic
+    samples: the row each sample hits gets one of five rows that sit on the
+    boundaries of the rule of SamplerGpuBasic.cu:28-45 (take the alias iff
+    row.w < W/N and coin * W/N >= row.w; an alias that is not an item keeps
+    the row).  Returns the rows and, for every sample alone on its row, the
+    item the rule must return."""
+    rw = np.ones(n)
+    ra = np.arange(n, dtype=np.int32)
+    hits = [basic_row_and_coin(oracle, seed, k, n) for k in range(count)]
+    rows = [r for r, _ in hits]
+    expected = {}
+    for k, (r, coin) in enumerate(hits):
+        if rows.count(r) > 1:
+            continue
+        alias = (r + 1 + k) % n
+        cat = k % 5
+        if cat == 0:    # coin * W/N == row.w exactly: alias (>=)
+            rw[r], ra[r], exp = coin, alias, alias
+        elif cat == 1:  # row.w one ulp above coin * W/N: row
+            rw[r], ra[r], exp = np.nextafter(coin, 2.0), alias, r
+        elif cat == 2:  # row.w == W/N: row (strict <)
+            rw[r], ra[r], exp = 1.0, alias, r
+        elif cat == 3:  # always-alias weight but the alias is the heavy sentinel N: row
+            rw[r], ra[r], exp = 0.0, n, r
+        else:           # row.w = 0: alias whatever the coin
+            rw[r], ra[r], exp = 0.0, alias, alias
+        expected[k] = exp
+    return rw, ra, expected
+
+
+def test_basic_sampler_decision_boundaries(oracle):
+    n, seed, count = 1 << 16, 77, 200
+    rw, ra, expected = decision_boundary_table(oracle, n, seed, count)
+    assert len(expected) > 150
+    out = oracle.sample_basic(rw, ra, 1.0, seed, count)
+    for k, exp in expected.items():
+        assert out[k] == exp, (k, k % 5, out[k], exp)
 
 
 @pytest.mark.parametrize("k", [10, 14, 16])
@@ -291,3 +416,56 @@ def test_rejection_array_brute_force(oracle):
     counts = np.floor((len(w) * w) / W).astype(np.int64) + 1
     assert np.array_equal(incl, np.cumsum(counts))
     assert np.array_equal(A, np.repeat(np.arange(len(w)), counts))
+
+
+# ---- batches (test infrastructure for the engine's batched build) ------------
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_oracle_equals_single_builds(oracle, dtype):
+    sizes = [1, 2, 17, 100, 1000, 4097, 20011, 65536]
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    w = np.concatenate([atg.generate_weights("exponential", n, seed=5 + t) for t, n in enumerate(sizes)]).astype(dtype)
+    kw = dict(chains=32, section_size=2048, num_splits=-38)
+    rw, ra, W = oracle.build_batch(w, offsets, **kw)
+    for t, n in enumerate(sizes):
+        lo, hi = offsets[t], offsets[t + 1]
+        ref = oracle.build(w[lo:hi].astype(np.float64), chains=32, section_size=2048,
+                           num_splits=max(2, n // 38), want_lists=False)
+        assert W[t] == ref["stats"]["W"]
+        assert np.array_equal(rw[lo:hi], ref["row_w"]) and np.array_equal(ra[lo:hi], ref["row_alias"]), t
+    # a subset leaves the other tables untouched
+    rw2, ra2, W2 = oracle.build_batch(w, offsets, which=[7, 3], **kw)
+    for t in range(len(sizes)):
+        lo, hi = offsets[t], offsets[t + 1]
+        if t in (3, 7):
+            assert np.array_equal(rw2[lo:hi], rw[lo:hi]) and W2[t] == W[t]
+        else:
+            assert np.all(ra2[lo:hi] == -1)
+    Wc, ok, mabs = oracle.check_batch(w, offsets, rw, ra, W, **kw)
+    assert np.array_equal(Wc, W) and ok.all()
+    # a changed alias in one table fails that table's check only
+    t = 6
+    ref6 = {"row_w": rw[offsets[t]:offsets[t + 1]], "row_alias": ra[offsets[t]:offsets[t + 1]],
+            "stats": {"W_N": W[t] / sizes[t]}}
+    bw, ba = corrupt_alias(ref6)
+    ra_bad = ra.copy()
+    ra_bad[offsets[t]:offsets[t + 1]] = ba
+    _, ok, _ = oracle.check_batch(w, offsets, rw, ra_bad, W, **kw)
+    assert not ok[t] and ok.sum() == len(sizes) - 1
+
+
+def test_sectioned_subset_equals_full_run(oracle):
+    n = 100_003
+    w = atg.generate_weights("pareto", n, seed=2, param=1.1)
+    ref = oracle.build(w, want_lists=False)
+    total, rows, seed = 3_000_000, 4096, 5
+    full = oracle.sample_sectioned(ref["row_w"], ref["row_alias"], ref["stats"]["W_N"], rows, total, seed)
+    nsec = (n + rows - 1) // rows
+    sel = [0, 5, nsec - 1, 11]
+    out, first, cnt = oracle.sample_sectioned_subset(ref["row_w"], ref["row_alias"], ref["stats"]["W_N"], rows,
+                                                     total, seed, sel)
+    pos = 0
+    for f, c in zip(first.tolist(), cnt.tolist()):
+        assert np.array_equal(out[pos:pos + c], full[f:f + c])
+        pos += c
+    assert pos == len(out)
