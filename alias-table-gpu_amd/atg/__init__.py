@@ -219,10 +219,14 @@ class Table:
         check(lib().atg_build(self.h, _stream(stream)))
 
     PHASES = ("sum", "partition", "split", "pack")
+    _PHASE_IDS = {"sum": 0, "partition": 1, "split": 2, "pack": 3, "partition_local": 4, "partition_place": 5}
 
     def build_phase(self, phase, stream=None):
-        """One phase of build(): 'sum', 'partition' (PSA+ and prefix sums), 'split' or 'pack'."""
-        check(lib().atg_build_phase(self.h, self.PHASES.index(phase), _stream(stream)))
+        """One phase of build(): 'sum', 'partition' (PSA+ and prefix sums), 'split' or 'pack';
+        'partition' also runs as its two passes 'partition_local' then 'partition_place'."""
+        if phase not in self._PHASE_IDS:
+            raise AtgError(_EINVAL, f"build_phase: unknown phase {phase!r}")
+        check(lib().atg_build_phase(self.h, self._PHASE_IDS[phase], _stream(stream)))
 
     def sync(self, stream=None):
         check(lib().atg_sync(self.h, _stream(stream)))
@@ -411,14 +415,14 @@ class Batch:
         _device_buffer(out, "i32", self.ntables * int(per_table), "Batch.sample: out")
         check(lib().atg_batch_sample(self.h, int(per_table), int(seed), _ptr(out), _stream(stream)))
 
-    def autotune(self, reps=3, makespan_slack=0.05):
-        """Tunes the split method and the launch groups in place
+    def autotune(self, reps=3, makespan_slack=0.05, tune_split=True):
+        """Tunes the launch groups and (tune_split) the split method in place
         (atg_batch_autotune); returns the chosen split and group size, the
         median makespan and the 99th-percentile table latency in ms."""
         o = BuildOpts()
         ms, p99 = C.c_double(0.0), C.c_double(0.0)
-        check(lib().atg_batch_autotune(self.h, int(reps), float(makespan_slack), C.byref(o), C.byref(ms),
-                                       C.byref(p99)))
+        check(lib().atg_batch_autotune(self.h, int(reps), 1 if tune_split else 0, float(makespan_slack), C.byref(o),
+                                       C.byref(ms), C.byref(p99)))
         return {"split": "learned" if o.split_method == ATG_SPLIT_LEARNED else "bisect",
                 "group_items": int(o.batch_group_items), "makespan_ms": ms.value, "p99_latency_ms": p99.value}
 

@@ -79,7 +79,7 @@ SIGNATURES = [
     ("atg_autotune", _I32, [_P, _I32, _I64, _I32, _I32, C.POINTER(BuildOpts), _I32, C.POINTER(BuildOpts), C.POINTER(_D)]),
     ("atg_autotune_shape", _I32, [_P, _I32, _I64, _I32, _I32, C.POINTER(BuildOpts), _I32, C.POINTER(BuildOpts),
                                   C.POINTER(_D)]),
-    ("atg_batch_autotune", _I32, [_P, _I32, _D, C.POINTER(BuildOpts), C.POINTER(_D), C.POINTER(_D)]),
+    ("atg_batch_autotune", _I32, [_P, _I32, _I32, _D, C.POINTER(BuildOpts), C.POINTER(_D), C.POINTER(_D)]),
     ("atg_batch_create", _I32, [C.POINTER(_P), _P, _I64, _I32, C.POINTER(BuildOpts)]),
     ("atg_batch_destroy", None, [_P]),
     ("atg_batch_set_weights", _I32, [_P, _P, _I32, _P]),

@@ -310,6 +310,8 @@ atg_status atg_build_phase(atg_table* t, int32_t phase, void* stream) {
     switch (phase) {
         case ATG_PHASE_SUM: HIPCHK(launch_sum(a, s)); break;
         case ATG_PHASE_PARTITION: HIPCHK(launch_partition(a, s)); break;
+        case ATG_PHASE_PARTITION_LOCAL: HIPCHK(launch_partition_local(a, s)); break;
+        case ATG_PHASE_PARTITION_PLACE: HIPCHK(launch_partition_place(a, s)); break;
         case ATG_PHASE_SPLIT:
             if (t->opts.split_method == ATG_SPLIT_LEARNED) HIPCHK(launch_split_learned(a, s));
             else HIPCHK(launch_split(a, s));

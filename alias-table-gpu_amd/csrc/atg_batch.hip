@@ -715,14 +715,14 @@ atg_status atg_batch_groups(atg_batch* b, int32_t* group_of_table, int64_t* ngro
 }
 
 // Times the table-neutral knobs of a batch in place: the split method
-// (bisection or the learned index) and the launch groups (one group, or
+// (bisection or the learned index; with tune_split) and the launch groups (one group, or
 // groups of total/2^k items down to about 2^26 items).  Each candidate is the
 // median makespan of `reps` builds after a warm-up build; among the
 // candidates whose makespan is within (1 + makespan_slack) of the fastest, the
 // one with the lowest 99th-percentile table latency wins.  The batch keeps the
 // winning configuration (any captured graph is dropped).
-atg_status atg_batch_autotune(atg_batch* b, int32_t reps, double makespan_slack, atg_build_opts* chosen,
-                              double* makespan_ms, double* p99_latency_ms) {
+atg_status atg_batch_autotune(atg_batch* b, int32_t reps, int32_t tune_split, double makespan_slack,
+                              atg_build_opts* chosen, double* makespan_ms, double* p99_latency_ms) {
     if (!b) return fail(ATG_EINVAL, "atg_batch_autotune: NULL batch");
     if (!b->weights_set) return fail(ATG_ESTATE, "atg_batch_autotune: weights not set");
     if (reps < 1) reps = 3;
@@ -738,7 +738,9 @@ atg_status atg_batch_autotune(atg_batch* b, int32_t reps, double makespan_slack,
     HIPCHK(hipEventCreate(&e1));
     atg_status rc = ATG_OK;
     std::vector<double> lat((size_t)b->ntables);
-    for (int32_t split : {ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED}) {
+    std::vector<int32_t> splits = {b->opts.split_method};
+    if (tune_split) splits = {ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED};
+    for (int32_t split : splits) {
         if (rc != ATG_OK) break;
         rc = batch_set_split(b, split);
         for (int64_t gi : gsizes) {

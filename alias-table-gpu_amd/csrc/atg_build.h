@@ -78,6 +78,8 @@ int64_t block_tree_total(int64_t G);
 hipError_t set_partition_lds_limit(size_t bytes);
 hipError_t launch_sum(const BuildArgs& a, hipStream_t s);
 hipError_t launch_partition(const BuildArgs& a, hipStream_t s);
+hipError_t launch_partition_local(const BuildArgs& a, hipStream_t s);
+hipError_t launch_partition_place(const BuildArgs& a, hipStream_t s);
 hipError_t launch_degenerate(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split_learned(const BuildArgs& a, hipStream_t s);

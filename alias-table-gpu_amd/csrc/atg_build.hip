@@ -130,12 +130,25 @@ int64_t block_tree_total(int64_t G) {
     return total;
 }
 
-hipError_t launch_partition(const BuildArgs& a, hipStream_t s) {
+// The partition phase in its two passes: the local pass (partition, PSA+
+// chains, staged leftovers, level-0 block sums), then the block tree and the
+// placement of the leftover lists (and the split certificate).
+hipError_t launch_partition_local(const BuildArgs& a, hipStream_t s) {
     const size_t lds = psa_lds_bytes(a.Ls_max);
     void (*k)(BuildArgs) = a.dtype_f32 ? (a.rows ? partition_local<float, true> : partition_local<float, false>)
                                        : (a.rows ? partition_local<double, true> : partition_local<double, false>);
     const dim3 g((unsigned)std::min<int64_t>(a.G, resident_blocks((const void*)k, 64, lds)));
     hipLaunchKernelGGL(k, g, dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_partition(const BuildArgs& a, hipStream_t s) {
+    const hipError_t e = launch_partition_local(a, s);
+    if (e != hipSuccess) return e;
+    return launch_partition_place(a, s);
+}
+
+hipError_t launch_partition_place(const BuildArgs& a, hipStream_t s) {
     for (int b0 = 0; (a.G >> b0) > 1; b0 += 9)
         hipLaunchKernelGGL(block_tree, dim3(blocks_for(a.G >> b0, 512)), dim3(256), 0, s, a, b0);
     hipLaunchKernelGGL(partition_place, dim3(blocks_for(a.G, 4)), dim3(256), 0, s, a);

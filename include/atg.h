@@ -117,7 +117,13 @@ atg_status atg_build(atg_table* t, void* stream);
  * prefix sums), SPLIT, PACK: the reference exposes performSplit/performPack
  * the same way (AliasTableSplitGpu.cuh:107-115) and times the phases with
  * TableSplitTimer (utils/Timer.cuh:17-45).  atg_build == all four in order. */
-typedef enum { ATG_PHASE_SUM = 0, ATG_PHASE_PARTITION = 1, ATG_PHASE_SPLIT = 2, ATG_PHASE_PACK = 3 } atg_phase;
+typedef enum {
+    ATG_PHASE_SUM = 0, ATG_PHASE_PARTITION = 1, ATG_PHASE_SPLIT = 2, ATG_PHASE_PACK = 3,
+    /* ATG_PHASE_PARTITION in its two passes, in this order (for per-kernel
+     * timing): the local pass (partition, PSA+ chains, staged leftovers), then
+     * the section prefixes and the placement of the leftover lists. */
+    ATG_PHASE_PARTITION_LOCAL = 4, ATG_PHASE_PARTITION_PLACE = 5
+} atg_phase;
 atg_status atg_build_phase(atg_table* t, int32_t phase, void* stream);
 
 /* Waits for the stream and returns the build's device-side status. */
@@ -294,13 +300,14 @@ atg_status atg_autotune(const void* weights, atg_mem mem, int64_t n, atg_dtype d
 atg_status atg_autotune_shape(const void* weights, atg_mem mem, int64_t n, atg_dtype dtype, atg_layout layout,
                               const atg_build_opts* base, int32_t reps, atg_build_opts* best, double* best_ms);
 /* Tunes a batch in place over the knobs that leave its tables unchanged: the
- * split method and the launch groups (batch_group_items).  Among the
+ * launch groups (batch_group_items) and, with tune_split = 1, the split
+ * method (0 keeps the batch's).  Among the
  * candidates whose median makespan is within (1 + makespan_slack) of the
  * fastest, the one with the lowest 99th-percentile table latency wins and
  * stays configured; *chosen receives the batch's options.  Weights must be
  * set.  No counterpart in the reference. */
-atg_status atg_batch_autotune(atg_batch* b, int32_t reps, double makespan_slack, atg_build_opts* chosen,
-                              double* makespan_ms, double* p99_latency_ms);
+atg_status atg_batch_autotune(atg_batch* b, int32_t reps, int32_t tune_split, double makespan_slack,
+                              atg_build_opts* chosen, double* makespan_ms, double* p99_latency_ms);
 
 /* ---- misc --------------------------------------------------------------- */
 const char* atg_last_error(void);
