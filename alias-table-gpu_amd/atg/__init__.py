@@ -142,6 +142,30 @@ def autotune_shape(w, dtype="f64", layout="aos", psa_plus=True, split="bisect", 
             "split": split, "psa_plus": psa_plus}, ms.value
 
 
+def build_noalloc(w, rows, aliases=None, W_out=None, error_out=None, dtype="f64", layout="aos", psa_plus=True,
+                  chains=16, section_size=0, num_splits=0, split="bisect", learned_segments=4096, stream=None):
+    """Builds the table of the device weights `w` into caller-owned device
+    buffers with no allocation (atg_build_noalloc): AoS rows as a float64
+    tensor of 2 n elements (16-byte rows), or SoA row weights (float64, n) and
+    `aliases` (int32, n).  W_out: float64 device tensor of 1 element or None;
+    error_out: int32 device tensor of 1 element or None."""
+    n = w.numel() if _is_device(w) else 0
+    _device_buffer(w, dtype, n, "build_noalloc: weights", exact=True)
+    if layout == "aos":
+        _device_buffer(rows, "f64", 2 * n, "build_noalloc: rows", exact=True)
+    else:
+        _device_buffer(rows, "f64", n, "build_noalloc: weights out", exact=True)
+        if aliases is None:
+            raise AtgError(_EINVAL, "build_noalloc: SoA needs aliases")
+        _device_buffer(aliases, "i32", n, "build_noalloc: aliases", exact=True)
+    _device_buffer(W_out, "f64", 1, "build_noalloc: W_out")
+    _device_buffer(error_out, "i32", 1, "build_noalloc: error_out")
+    o = _shape_opts(psa_plus, chains, section_size, num_splits, split, learned_segments)
+    check(lib().atg_build_noalloc(_ptr(w), ATG_F32 if dtype == "f32" else ATG_F64, int(n),
+                                  ATG_SOA if layout == "soa" else ATG_AOS, C.byref(o), _ptr(rows), _ptr(aliases),
+                                  _ptr(W_out), _ptr(error_out), _stream(stream)))
+
+
 def write_table_file(path, info, w, alias):
     """Writes a table file (atg_table_file_write): info is a dict with n, W and
     optionally the build shape (layout, source_dtype, psa_plus, chains,

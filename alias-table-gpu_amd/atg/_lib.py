@@ -91,6 +91,7 @@ SIGNATURES = [
     ("atg_batch_sample", _I32, [_P, _U64, _U64, _P, _P]),
     ("atg_batch_latencies", _I32, [_P, _P]),
     ("atg_batch_use_graph", _I32, [_P, _I32]),
+    ("atg_build_noalloc", _I32, [_P, _I32, _I64, _I32, C.POINTER(BuildOpts), _P, _P, _P, _P, _P]),
     ("atg_table_file_write", _I32, [C.c_char_p, C.POINTER(TableFileInfo), _P, _P]),
     ("atg_table_file_read", _I32, [C.c_char_p, C.POINTER(TableFileInfo), _P, _P, _I64]),
     ("atg_save_table", _I32, [_P, C.c_char_p, _U64]),

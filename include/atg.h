@@ -154,6 +154,24 @@ atg_status atg_get_splits(atg_table* t, int32_t* i, int32_t* j, double* spill, i
 atg_status atg_get_lh(atg_table* t, int32_t* L_item, double* L_w, double* L_pref, int32_t* H_item,
                       double* H_w, double* H_pref);
 
+/* A build that allocates nothing: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Builds the table of
+ * n device weights (dtype) into the caller's device rows, exactly as
+ * atg_create + atg_set_weights(ATG_MEM_DEVICE) + atg_build with the same
+ * options build it: AoS rows {double weight; int32 alias; int32 0} at
+ * rows_or_weights, or SoA weights at rows_or_weights and aliases at aliases.
+ * W_out (device, nullable) receives W; error_out (device, nullable) the
+ * device-side error bits (0: valid weights).  Every scratch buffer lives in
+ * one per-device array of ATG_NOALLOC_SCRATCH_BYTES, held until the process
+ * ends; shapes that need more (about n > 2^20 at the default sections) are
+ * rejected with ATG_EINVAL.  interleave and record_phases are not supported.
+ * Calls that overlap on one device race on that array. */
+#define ATG_NOALLOC_SCRATCH_BYTES (64u << 20)
+atg_status atg_build_noalloc(const void* weights, atg_dtype dtype, int64_t n, atg_layout layout,
+                             const atg_build_opts* opts, void* rows_or_weights, int32_t* aliases,
+                             double* W_out, uint32_t* error_out, void* stream);
+
 /* ---- table files ------------------------------------------------------ */
 
 /* Header of a versioned table file (no counterpart in the reference, whose
