@@ -333,6 +333,12 @@ class Table:
         check(lib().atg_rejection_size(self.h, C.byref(v)))
         return v.value
 
+    def rejection_weight_sum(self):
+        """W the rejection structure was built with (atg_rejection_weight_sum)."""
+        v = C.c_double(0.0)
+        check(lib().atg_rejection_weight_sum(self.h, C.byref(v)))
+        return v.value
+
     def rejection_array(self):
         sz = self.rejection_size()
         A = np.empty(sz, np.int32)

@@ -72,6 +72,7 @@ SIGNATURES = [
     ("atg_sample", _I32, [_P, _I32, _U64, _U64, _I64, _P, _P, C.POINTER(_U64), _P]),
     ("atg_rejection_build", _I32, [_P, _P]),
     ("atg_rejection_size", _I32, [_P, C.POINTER(_U64)]),
+    ("atg_rejection_weight_sum", _I32, [_P, C.POINTER(_D)]),
     ("atg_get_rejection_array", _I32, [_P, _P, _U64]),
     ("atg_rejection_sample", _I32, [_P, _U64, _U64, _P, _P, C.POINTER(_U64), _P]),
     ("atg_generate_weights", _I32, [_I32, _I64, _U64, _D, _P]),

@@ -77,6 +77,9 @@ struct Variant {
     int chains = 16;
     int64_t sectionSize = 0;
     int learnedSegments = 4096;
+    // performSplitAndPackInterleaved (AliasTableSplitGpu.cu:220-262): split
+    // and pack on two streams; no per-phase timers then.
+    bool interleave = false;
 };
 
 template <Storage S = Storage::ArrayOfStructs>
@@ -90,6 +93,7 @@ class AliasTableSplitGpu : public SamplingAlgorithm {
     }
     std::string name() override {
         std::string v = variant_.split == ATG_SPLIT_LEARNED ? "LearnedSplit" : "";
+        if (variant_.interleave) v += "Interleaved";
         if (variant_.psaPlus) v += "PsaPlus";
         return std::string("AliasTableSplitGpu") +
                (S == Storage::ArrayOfStructs ? "ArrayOfStructs" : "StructOfArrays") + v;
@@ -106,7 +110,8 @@ class AliasTableSplitGpu : public SamplingAlgorithm {
             o.num_splits = numThreads_ == NUMBER_THREADS_AUTO ? 0 : numThreads_;
             o.split_method = variant_.split;
             o.learned_segments = variant_.learnedSegments;
-            o.record_phases = 1;
+            o.interleave = variant_.interleave ? 1 : 0;
+            o.record_phases = variant_.interleave ? 0 : 1;
             check(atg_create(&table_, N, ATG_F64, S == Storage::ArrayOfStructs ? ATG_AOS : ATG_SOA, &o));
         }
         check(atg_set_weights(table_, weights.data(), ATG_MEM_HOST, nullptr));
@@ -161,9 +166,7 @@ class RejectionSampling : public SamplingAlgorithm {
     void build() override {
         check(atg_rejection_build(table_, nullptr));
         check(atg_sync(table_, nullptr));
-        atg_stats st;
-        check(atg_get_stats(table_, &st));
-        W = st.W;
+        check(atg_rejection_weight_sum(table_, &W));
     }
     // postBuild copies A to the host (RejectionSampling.cu:19-24).
     bool postBuild() override {

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <random>
 
 #include "atg.hpp"
 
@@ -54,6 +55,34 @@ static void testTableCorrectness() {
     expect_build<Storage::ArrayOfStructs>(505711, v, 16576, weightDistributionOneLight);
 }
 
+// The random rounds of UnitTests::testTableCorrectness (UnitTests.cu:51-93):
+// 20 rounds, each a random variant, n = 100 + a random number below 10^e
+// with e uniform in 3..7 (small tables as likely as large ones), a random
+// weight distribution and a random PSA+ choice, built as AoS; then the same
+// with PSA+ off as SoA.  The reference draws from an unseeded rand(); here
+// a fixed seed makes a failure reproducible.
+static void testRandomRounds(uint32_t seed) {
+    std::mt19937 rng(seed);
+    struct V { atg_split_method split; bool interleave; bool psa_ok; };
+    const V variants[] = {{ATG_SPLIT_BISECT, false, true}, {ATG_SPLIT_LEARNED, false, true},
+                          {ATG_SPLIT_BISECT, true, true}, {ATG_SPLIT_LEARNED, true, true}};
+    const int numTests = 20;
+    std::printf("random rounds (seed %u)\n", seed);
+    for (int i = 0; i < numTests; ++i) {
+        const V& pick = variants[rng() % (sizeof(variants) / sizeof(V))];
+        const int64_t n_max = (int64_t)std::pow(10.0, 3 + (int)(rng() % 5));  // 1e3 .. 1e7
+        const int64_t n = 100 + (int64_t)(rng() % (uint64_t)n_max);
+        const WeightDistribution d = (WeightDistribution)(rng() % MAX_WEIGHT_DISTRIBUTION + 1);
+        Variant v;
+        v.split = pick.split;
+        v.interleave = pick.interleave;
+        v.psaPlus = pick.psa_ok && (rng() % 2) == 1;
+        expect_build<Storage::ArrayOfStructs>(n, v, NUMBER_THREADS_AUTO, d);
+        v.psaPlus = false;
+        expect_build<Storage::StructOfArrays>(n, v, NUMBER_THREADS_AUTO, d);
+    }
+}
+
 // UnitTests::testSamplingDistribution (UnitTests.cu:96-145).
 static void testSamplingDistribution() {
     const uint64_t numSamples = 2000000;
@@ -97,6 +126,7 @@ static void testSamplingDistribution() {
 int main() {
     try {
         testTableCorrectness();
+        testRandomRounds(20260101u);
         testSamplingDistribution();
     } catch (const Error& e) {
         std::printf("FAILED with status %d: %s\n", (int)e.status, e.what());

@@ -728,6 +728,16 @@ atg_status atg_rejection_size(atg_table* t, uint64_t* a_size) {
     return state_status(st);
 }
 
+atg_status atg_rejection_weight_sum(atg_table* t, double* W) {
+    if (!t || !W) return fail(ATG_EINVAL, "atg_rejection_weight_sum: NULL argument");
+    if (!t->rej_built) return fail(ATG_ESTATE, "atg_rejection_weight_sum: rejection structure not built");
+    DevState st;
+    HIPCHK(hipStreamSynchronize(t->last_stream));
+    HIPCHK(hipMemcpy(&st, t->d_rej_state, sizeof(st), hipMemcpyDeviceToHost));
+    *W = st.W;
+    return state_status(st);
+}
+
 atg_status atg_get_rejection_array(atg_table* t, int32_t* A, uint64_t cap) {
     uint64_t sz = 0;
     atg_status rc = atg_rejection_size(t, &sz);

@@ -231,6 +231,9 @@ atg_status atg_sample(atg_table* t, atg_sample_kind kind, uint64_t count, uint64
  * canonical W.  Requires atg_set_weights (a table build is not needed). */
 atg_status atg_rejection_build(atg_table* t, void* stream);
 atg_status atg_rejection_size(atg_table* t, uint64_t* a_size);
+/* RejectionSampling::W (SamplingAlgorithm.cuh:44): the canonical weight sum
+ * the rejection structure was built with (kept apart from the alias build's). */
+atg_status atg_rejection_weight_sum(atg_table* t, double* W);
 atg_status atg_get_rejection_array(atg_table* t, int32_t* A_host, uint64_t capacity);
 
 /* SamplerRejection::executeBenchmarkSampling (SamplerRejection.cu:63-72). */

@@ -135,6 +135,7 @@ def test_rejection_and_basic_headline(gpu, oracle, headline):
     t.rejection_build()
     A = t.rejection_array()
     W = ref["stats"]["W"]
+    assert t.rejection_weight_sum() == W and t.stats()["W"] == W
     _, A_ref = oracle.rejection_build(w, W)
     assert A.size == A_ref.size and np.array_equal(A, A_ref)
     count = 10_000_000
