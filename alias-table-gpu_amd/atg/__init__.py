@@ -333,6 +333,16 @@ class Table:
         check(lib().atg_rejection_size(self.h, C.byref(v)))
         return v.value
 
+    def rejection_sample_sectioned(self, count, seed, section_slots=4096, out=None, hist=None, checksum=False,
+                                   stream=None):
+        """Rejection sampling by LDS-staged sections of A (atg_rejection_sample_sectioned)."""
+        _device_buffer(out, "i32", int(count), "rejection_sample_sectioned: out")
+        _device_buffer(hist, "u32", self.n, "rejection_sample_sectioned: hist")
+        cs = C.c_uint64(0)
+        check(lib().atg_rejection_sample_sectioned(self.h, int(count), int(seed), int(section_slots), _ptr(out),
+                                                   _ptr(hist), C.byref(cs) if checksum else None, _stream(stream)))
+        return cs.value if checksum else None
+
     def rejection_weight_sum(self):
         """W the rejection structure was built with (atg_rejection_weight_sum)."""
         v = C.c_double(0.0)

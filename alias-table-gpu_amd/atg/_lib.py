@@ -75,6 +75,7 @@ SIGNATURES = [
     ("atg_rejection_weight_sum", _I32, [_P, C.POINTER(_D)]),
     ("atg_get_rejection_array", _I32, [_P, _P, _U64]),
     ("atg_rejection_sample", _I32, [_P, _U64, _U64, _P, _P, C.POINTER(_U64), _P]),
+    ("atg_rejection_sample_sectioned", _I32, [_P, _U64, _U64, _I64, _P, _P, C.POINTER(_U64), _P]),
     ("atg_generate_weights", _I32, [_I32, _I64, _U64, _D, _P]),
     ("atg_generate_batch", _I32, [_I32, _I64, _I64, _I64, _U64, _D, _P, _P, _I32]),
     ("atg_autotune", _I32, [_P, _I32, _I64, _I32, _I32, C.POINTER(BuildOpts), _I32, C.POINTER(BuildOpts), C.POINTER(_D)]),

@@ -85,6 +85,15 @@ struct atg_table_s {
     uint64_t A_cap = 0;
     uint64_t *d_rej_tot = nullptr, *d_rej_off = nullptr;
     RejTask* d_rej_tasks = nullptr;
+    // sectioned rejection sampling: |A| read back once per rejection build;
+    // section masses, their prefix and the section counts for rej_sec_cap
+    // sections, the masses valid for rej_sec_SS slots per section
+    uint64_t rej_size_host = 0;
+    bool rej_size_known = false;
+    double *d_rej_mass = nullptr, *d_rej_pre = nullptr;
+    uint64_t* d_rej_sec_counts = nullptr;
+    int64_t rej_sec_cap = 0, rej_sec_SS = 0;
+    bool rej_sec_valid = false;
     // sectioned sampling: count and first output of each section
     uint64_t* d_sec_counts = nullptr;
     int64_t sec_counts_cap = 0;
@@ -165,7 +174,7 @@ static void free_table(atg_table* t) {
                     t->d_stage_i, t->d_Li, t->d_Hi,
                     t->d_Lw, t->d_Lp, t->d_Hw, t->d_Hp, t->d_rows, t->d_tw, t->d_ta, t->d_splits, t->d_knots,
                     t->d_checksum, t->d_rej_state, t->d_A, t->d_rej_tot, t->d_rej_off, t->d_rej_tasks,
-                    t->d_sec_counts};
+                    t->d_sec_counts, t->d_rej_mass, t->d_rej_pre, t->d_rej_sec_counts};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->events)
@@ -715,6 +724,80 @@ atg_status atg_rejection_build(atg_table* t, void* stream) {
     r.tasks_cap = rejection_task_cap(t->A_cap);
     HIPCHK(launch_rejection_build(r, s));
     t->rej_built = true;
+    t->rej_size_known = false;
+    t->rej_sec_valid = false;
+    return ATG_OK;
+}
+
+atg_status atg_rejection_sample_sectioned(atg_table* t, uint64_t count, uint64_t seed, int64_t section_slots,
+                                          int32_t* out, uint32_t* hist, uint64_t* checksum, void* stream) {
+    if (!t) return fail(ATG_EINVAL, "atg_rejection_sample_sectioned: NULL table");
+    if (!t->rej_built) return fail(ATG_ESTATE, "atg_rejection_sample_sectioned: rejection structure not built");
+    if (section_slots <= 0) section_slots = 4096;
+    if (section_slots > 8192)
+        return fail(ATG_EINVAL, "atg_rejection_sample_sectioned: section_slots %lld > 8192", (long long)section_slots);
+    hipStream_t s = (hipStream_t)stream;
+    if (!t->rej_size_known) {
+        // |A| sizes the launch: read back once per rejection build.
+        HIPCHK(hipStreamSynchronize(t->last_stream));
+        DevState st;
+        HIPCHK(hipMemcpy(&st, t->d_rej_state, sizeof(st), hipMemcpyDeviceToHost));
+        const atg_status rc = state_status(st);
+        if (rc != ATG_OK) return rc;
+        t->rej_size_host = st.rej_size;
+        t->rej_size_known = true;
+    }
+    t->last_stream = s;
+    const uint64_t Asz = t->rej_size_host;
+    if (Asz == 0 || Asz > t->A_cap) return fail(ATG_ESTATE, "atg_rejection_sample_sectioned: |A| = %llu", (unsigned long long)Asz);
+    const int64_t nsec = (int64_t)((Asz + (uint64_t)section_slots - 1) / (uint64_t)section_slots);
+    if (nsec > t->rej_sec_cap) {
+        // Grows only; freed with the table.
+        HIPCHK(hipStreamSynchronize(s));
+        void* ptrs[] = {t->d_rej_mass, t->d_rej_pre, t->d_rej_sec_counts};
+        for (void* p : ptrs)
+            if (p) HIPCHK(hipFree(p));
+        t->d_rej_mass = t->d_rej_pre = nullptr;
+        t->d_rej_sec_counts = nullptr;
+        t->rej_sec_cap = 0;
+        if (dalloc(&t->d_rej_mass, (size_t)nsec) != hipSuccess || dalloc(&t->d_rej_pre, (size_t)nsec + 1) != hipSuccess ||
+            dalloc(&t->d_rej_sec_counts, 2 * (size_t)nsec) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(ATG_ENOMEM, "atg_rejection_sample_sectioned: section buffers");
+        }
+        t->rej_sec_cap = nsec;
+        t->rej_sec_valid = false;
+    }
+    const bool masses = !t->rej_sec_valid || t->rej_sec_SS != section_slots;
+    RejSecArgs r{};
+    r.A = t->d_A;
+    r.A_size = Asz;
+    r.SS = section_slots;
+    r.nsec = nsec;
+    r.mass = t->d_rej_mass;
+    r.pre = t->d_rej_pre;
+    r.sec_counts = t->d_rej_sec_counts;
+    SampleArgs a{};
+    a.st = t->d_rej_state;
+    a.n = t->n;
+    a.count = count;
+    a.seed = seed;
+    a.out = out;
+    a.hist = hist;
+    a.checksum = checksum ? t->d_checksum : nullptr;
+    if (checksum) HIPCHK(hipMemsetAsync(t->d_checksum, 0, 8, s));
+    if (count > 0) {
+        HIPCHK(set_rej_sample_lds_limit((int)(section_slots * sizeof(RejEntry))));
+        HIPCHK(launch_rejection_sample_sectioned(r, a, masses, s));
+        t->rej_sec_valid = true;
+        t->rej_sec_SS = section_slots;
+    }
+    if (checksum) {
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpyAsync(&v, t->d_checksum, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *checksum = v;
+    }
     return ATG_OK;
 }
 

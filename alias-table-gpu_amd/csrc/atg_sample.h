@@ -15,8 +15,13 @@ enum : uint32_t {
     PURPOSE_SECTION_COUNTS = 0x0000B003u,
     PURPOSE_REJECTION = 0x0000B004u,
     PURPOSE_BATCH = 0x0000B005u,
+    PURPOSE_REJ_SECTIONED = 0x0000B006u,
+    PURPOSE_REJ_SECTION_COUNTS = 0x0000B007u,
 };
 #define REJECTION_MAX_TRIALS 256
+// Trials of one sample of the sectioned rejection sampler: a section of
+// low-acceptance slots can need many, but it draws few samples.
+#define REJ_SECTIONED_MAX_TRIALS (1u << 20)
 
 struct SampleArgs {
     const Row* rows;       // AoS table or null
@@ -50,7 +55,21 @@ struct RejArgs {
     uint32_t tasks_cap;
 };
 
+// Sectioned rejection sampling over A: sections of SS slots.
+struct RejSecArgs {
+    const RejEntry* A;
+    uint64_t A_size;       // |A|
+    int64_t SS;            // slots per section
+    int64_t nsec;
+    double* mass;          // nsec section acceptance masses
+    double* pre;           // nsec + 1: their exclusive prefix
+    uint64_t* sec_counts;  // 2 nsec: count and first output of each section
+};
+
 hipError_t launch_sample_basic(const SampleArgs& a, hipStream_t s);
+// masses: recompute the section masses and their prefix (a new A or section size).
+hipError_t launch_rejection_sample_sectioned(const RejSecArgs& a, const SampleArgs& sa, bool masses, hipStream_t s);
+hipError_t set_rej_sample_lds_limit(int bytes);
 hipError_t launch_sample_sectioned(const SampleArgs& a, hipStream_t s);
 hipError_t set_sample_lds_limit(int bytes);
 hipError_t launch_rejection_build(const RejArgs& a, hipStream_t s);

@@ -71,13 +71,14 @@ __global__ __launch_bounds__(256) void sample_basic(SampleArgs a) {
 // Sectioned sampling.  Section counts: recursive halving of the section range
 // with a binomial draw per node (SamplerGpuSectioned.cu:58-96).  The reference
 // uses a float normal approximation; restated with IEEE + - * / sqrt floor only.
-__device__ uint64_t binomial(uint64_t n, double q, uint64_t seed, uint64_t node) {
+__device__ uint64_t binomial(uint64_t n, double q, uint64_t seed, uint64_t node,
+                             uint32_t purpose = PURPOSE_SECTION_COUNTS) {
     if (n == 0) return 0;
     if (n <= 32) {
         uint64_t cnt = 0;
         u32x4 x{0, 0, 0, 0};
         for (uint64_t t = 0; t < n; ++t) {
-            if ((t & 3) == 0) x = philox_at(node, PURPOSE_SECTION_COUNTS, (uint32_t)(t >> 2), seed);
+            if ((t & 3) == 0) x = philox_at(node, purpose, (uint32_t)(t >> 2), seed);
             const uint32_t word = ((t & 3) == 0) ? x.x : ((t & 3) == 1) ? x.y : ((t & 3) == 2) ? x.z : x.w;
             const double u = (double)word * 0x1.0p-32;
             if (u < q) ++cnt;
@@ -86,7 +87,7 @@ __device__ uint64_t binomial(uint64_t n, double q, uint64_t seed, uint64_t node)
     }
     double z = 0.0;
     for (uint32_t d = 0; d < 3; ++d) {
-        const u32x4 x = philox_at(node, PURPOSE_SECTION_COUNTS, d, seed);
+        const u32x4 x = philox_at(node, purpose, d, seed);
         z = z + (double)x.x * 0x1.0p-32;
         z = z + (double)x.y * 0x1.0p-32;
         z = z + (double)x.z * 0x1.0p-32;
@@ -452,6 +453,145 @@ __global__ __launch_bounds__(256) void rej_sample(RejArgs a, SampleArgs sa) {
 }
 
 // ---------------------------------------------------------------------------
+// Sectioned rejection sampling: the trials of SamplerRejection.cu:17-35 inside
+// sections of A, with the section counts of SamplerGpuSectioned.cu:58-136.  A
+// trial on slot s accepts iff coin53 <= accept(s), so its acceptance
+// probability is exactly P(s) = min(floor(accept(s) 2^53) + 1, 2^53) 2^-53.
+// An accepted sample lies in section k with probability M_k / sum M, M_k the
+// sum of P(s) over the section: the sample count of each section comes from
+// the halving tree with q from the exclusive prefix of the M_k, and inside a
+// section a sample repeats trials on uniform slots of the LDS-staged section.
+#define REJ_SEC_THREADS 1024
+
+__device__ __forceinline__ double rej_accept_prob(double accept) {
+    return accept >= 1.0 ? 1.0 : (floor(accept * 0x1.0p53) + 1.0) * 0x1.0p-53;
+}
+
+// Section masses, one wave per section: lane l adds slots l, l + 64, ... in
+// order, then the pairwise tree (the canonical section sum); 16 loads per
+// lane in flight.
+__global__ __launch_bounds__(256) void rej_section_mass(RejSecArgs a) {
+    const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= a.nsec) return;  // whole waves
+    const uint64_t s0 = (uint64_t)k * (uint64_t)a.SS, s1 = min(a.A_size, s0 + (uint64_t)a.SS);
+    double acc = 0.0;
+    for (uint64_t b = s0; b < s1; b += 16 * 64) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const uint64_t s = b + (uint64_t)u * 64 + lane;
+            v[u] = s < s1 ? a.A[s].accept : 1.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (b + (uint64_t)u * 64 + lane < s1) acc = acc + rej_accept_prob(v[u]);
+    }
+    const double m = wave_tree_sum(acc);
+    if (lane == 0) a.mass[k] = m;
+}
+
+// Exclusive prefix pre[0..nsec] of the section masses by one block, in the
+// canonical local-scan association: chunks of 64 masses scanned by the wave
+// scan, chunk carries added left to right (carry_{c+1} = carry_c + the
+// chunk's total), pre[64 c + l] = carry_c + the chunk's exclusive value.  In
+// tiles of REJ_SCAN_TILE chunks: the waves scan the tile's chunks and leave
+// their totals in LDS, one thread turns them into carries in order, then the
+// waves scan the chunks again and write the prefixes.
+#define REJ_SCAN_TILE 4096
+__global__ __launch_bounds__(1024) void rej_mass_scan(RejSecArgs a) {
+    __shared__ double tot[REJ_SCAN_TILE];
+    __shared__ double carry_in;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int64_t nch = (a.nsec + 63) / 64;
+    if (threadIdx.x == 0) carry_in = 0.0;
+    for (int64_t t0 = 0; t0 < nch; t0 += REJ_SCAN_TILE) {
+        const int cnt = (int)min<int64_t>(REJ_SCAN_TILE, nch - t0);
+        for (int c = wave; c < cnt; c += nw) {
+            const int64_t k = (t0 + c) * 64 + lane;
+            const double s = wave_scan(k < a.nsec ? a.mass[k] : 0.0);
+            if (lane == 63) tot[c] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double carry = carry_in;
+            for (int c = 0; c < cnt; ++c) {
+                const double x = tot[c];
+                tot[c] = carry;
+                carry = carry + x;
+            }
+            carry_in = carry;
+        }
+        __syncthreads();
+        for (int c = wave; c < cnt; c += nw) {
+            const int64_t k = (t0 + c) * 64 + lane;
+            const double s = wave_scan(k < a.nsec ? a.mass[k] : 0.0);
+            const double ex = tot[c] + wave_shr1(s);
+            if (k < a.nsec) a.pre[k] = ex;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.pre[a.nsec] = carry_in;
+}
+
+// Count and first output of every section, one thread per section.
+__global__ __launch_bounds__(64) void rej_section_counts(RejSecArgs a, uint64_t count, uint64_t seed) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nsec) return;
+    int64_t lo = 0, hi = a.nsec;
+    uint64_t c = count, o = 0, node = 1;
+    while (hi - lo > 1) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        // A range whose mass vanishes in the prefix's rounding splits by
+        // section count.
+        const double den = a.pre[hi] - a.pre[lo];
+        const double q = den > 0.0 ? (a.pre[mid] - a.pre[lo]) / den : (double)(mid - lo) / (double)(hi - lo);
+        const uint64_t x = binomial(c, q, seed, node, PURPOSE_REJ_SECTION_COUNTS);
+        if (k < mid) {
+            hi = mid; c = x; node = 2 * node;
+        } else {
+            lo = mid; o += x; c = c - x; node = 2 * node + 1;
+        }
+    }
+    a.sec_counts[2 * k] = c;
+    a.sec_counts[2 * k + 1] = o;
+}
+
+// One block per section: the section's slots staged in LDS, then every
+// sample of the section makes trials t = 0, 1, ... with the Philox block at
+// counter (output position, PURPOSE_REJ_SECTIONED, t): a uniform slot of the
+// section from words 0-1 and the coin from words 2-3.
+__global__ __launch_bounds__(REJ_SEC_THREADS) void rej_sample_sectioned(RejSecArgs a, SampleArgs sa) {
+    extern __shared__ RejEntry rsect[];
+    const int64_t k = blockIdx.x;
+    const uint64_t start = (uint64_t)k * (uint64_t)a.SS;
+    const uint64_t len = min(a.A_size, start + (uint64_t)a.SS) - start;
+    for (uint64_t s = threadIdx.x; s < len; s += blockDim.x)
+        *reinterpret_cast<uint4*>(&rsect[s]) = *reinterpret_cast<const uint4*>(&a.A[start + s]);
+    __syncthreads();
+    const uint64_t cnt = a.sec_counts[2 * k], off = a.sec_counts[2 * k + 1];
+    uint64_t csum = 0;
+    // One trial per iteration; a lane whose trial accepts goes on to its next
+    // sample at once, so no lane waits for another lane's rejected trials
+    // (the trials of every sample are those of the one-sample-at-a-time loop).
+    uint64_t t = threadIdx.x;
+    uint32_t tr = 0;
+    while (t < cnt) {
+        const uint64_t g = off + t;
+        const u32x4 r = philox_at(g, PURPOSE_REJ_SECTIONED, tr, sa.seed);
+        const uint4 v = *reinterpret_cast<const uint4*>(&rsect[scale64(word64(r), len)]);
+        const double acc = __longlong_as_double(((long long)v.y << 32) | v.x);
+        ++tr;
+        if (!(coin53(r) > acc) || tr >= REJ_SECTIONED_MAX_TRIALS) {
+            emit(sa, g, (int32_t)v.z, csum);
+            t += blockDim.x;
+            tr = 0;
+        }
+    }
+    flush_checksum(sa, csum);
+}
+
+// ---------------------------------------------------------------------------
 static inline unsigned grid_for(uint64_t count, unsigned block, unsigned cap) {
     uint64_t g = (count + block - 1) / block;
     if (g > cap) g = cap;
@@ -504,6 +644,21 @@ int64_t rejection_blocks(int64_t n) { return (n + REJ_ITEMS - 1) / REJ_ITEMS; }
 // <= (len - 1) / REJ_TASK + (len - 1) / REJ_WAVE tasks, so |A| <= A_cap slots
 // make at most A_cap (1 / REJ_TASK + 1 / REJ_WAVE) tasks.
 uint32_t rejection_task_cap(uint64_t A_cap) { return (uint32_t)(A_cap / REJ_TASK + A_cap / REJ_WAVE + 1); }
+
+hipError_t launch_rejection_sample_sectioned(const RejSecArgs& a, const SampleArgs& sa, bool masses, hipStream_t s) {
+    if (masses) {
+        hipLaunchKernelGGL(rej_section_mass, dim3((unsigned)((a.nsec + 3) / 4)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(rej_mass_scan, dim3(1), dim3(1024), 0, s, a);
+    }
+    hipLaunchKernelGGL(rej_section_counts, dim3((unsigned)((a.nsec + 63) / 64)), dim3(64), 0, s, a, sa.count, sa.seed);
+    const size_t lds = (size_t)a.SS * sizeof(RejEntry);
+    hipLaunchKernelGGL(rej_sample_sectioned, dim3((unsigned)a.nsec), dim3(REJ_SEC_THREADS), lds, s, a, sa);
+    return hipGetLastError();
+}
+
+hipError_t set_rej_sample_lds_limit(int bytes) {
+    return hipFuncSetAttribute((const void*)rej_sample_sectioned, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
 
 hipError_t launch_rejection_sample(const RejArgs& a, const SampleArgs& sa, hipStream_t s) {
     const unsigned g = grid_for(sa.count, 256, 256 * 64);

@@ -39,7 +39,10 @@ KERNEL_NAMES = {"sum": "sum_sections+sum_finish", "partition_local": "partition_
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE
 # (scaled by the counters of a device copy of known size) for the headline
 # workload at the given shape; the file holds the commands and the counts.
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {
+    (100_000_000, 2048, 24, 1_562_500, "partition_local"): (
+        2_297_000_000, "profiles/build_pareto1e8_s2048_c24_p1562500_hbm_traffic.md"),
+}
 
 
 def log(*a):
@@ -146,6 +149,20 @@ def bench_sampling(args, stream, t, n):
                 "rejection_ms": round(rms, 3), "rejection_gsamples_s": round(S / (rms * 1e-3) / 1e9, 3),
                 "rejection_hbm_frac": round(rbytes / (rms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
     log(f"[bench] rejection build {rb_ms:.3f} ms, sampling {rms:.3f} ms")
+    # By LDS-staged sections of A: the first call computes the section
+    # masses (kept for later calls with the same section size); timed apart.
+    slots = args.sample_section
+    fm, _ = time_events(stream, lambda: t.rejection_sample_sectioned(S, seed=4, section_slots=slots, out=out,
+                                                                     stream=stream), 1)
+    seeds = iter(range(200, 300))
+    sms, _ = time_events(stream, lambda: t.rejection_sample_sectioned(S, seed=next(seeds), section_slots=slots,
+                                                                      out=out, stream=stream), 3)
+    sbytes = 4 * S + 16 * A  # A read once, an int32 per sample
+    res.update({"rejection_sectioned_ms": round(sms, 3), "rejection_sectioned_first_call_ms": round(fm, 3),
+                "rejection_sectioned_gsamples_s": round(S / (sms * 1e-3) / 1e9, 3),
+                "rejection_sectioned_hbm_frac": round(sbytes / (sms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "rejection_section_slots": slots})
+    log(f"[bench] sectioned rejection sampling {sms:.3f} ms (first call {fm:.3f} ms)")
     del out
     torch.cuda.empty_cache()
     return res
@@ -368,6 +385,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sampling", action="store_true")
     ap.add_argument("--no-precision", action="store_true", help="skip the implied-probability checks")
+    ap.add_argument("--no-reference-shape", action="store_true", help="skip the build at the reference's default shape")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy a device buffer of this many MiB 3 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -445,8 +463,9 @@ def main():
         result["sampling"] = bench_sampling(args, stream, t, n)
     del t
     torch.cuda.empty_cache()
-    result["reference_shape"] = bench_reference_shape(args, stream, w)
-    log(f"[bench] reference shape: {result['reference_shape']}")
+    if not args.no_reference_shape:
+        result["reference_shape"] = bench_reference_shape(args, stream, w)
+        log(f"[bench] reference shape: {result['reference_shape']}")
     if not args.no_learned:
         result["learned"] = bench_learned(args, stream)
         if "precision" in result:

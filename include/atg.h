@@ -240,6 +240,19 @@ atg_status atg_get_rejection_array(atg_table* t, int32_t* A_host, uint64_t capac
 atg_status atg_rejection_sample(atg_table* t, uint64_t count, uint64_t seed, int32_t* out,
                                 uint32_t* hist, uint64_t* checksum, void* stream);
 
+/* Rejection sampling by sections of A (the trials of SamplerRejection.cu:17-35
+ * with the sections of SamplerGpuSectioned.cu:58-136; the reference has no
+ * sectioned rejection sampler).  A is cut into sections of section_slots
+ * slots (0 = 4096, at most 8192), each staged in LDS; a section's sample count
+ * follows its acceptance mass (halving tree of binomial draws, exact total);
+ * inside a section a sample repeats trials on uniform slots with the
+ * reference's acceptance rule.  Samples are written in section order.  The
+ * first call after atg_rejection_build reads |A| back (a wait for the
+ * stream); a call with more sections than any before grows the section
+ * buffers. */
+atg_status atg_rejection_sample_sectioned(atg_table* t, uint64_t count, uint64_t seed, int64_t section_slots,
+                                          int32_t* out, uint32_t* hist, uint64_t* checksum, void* stream);
+
 /* ---- input synthesis (SamplingAlgorithm::generateWeights, SamplingAlgorithm.cu:21-71) */
 
 typedef enum {

@@ -539,13 +539,13 @@ extern "C" void oracle_sample_batch_table(const double* row_w, const int32_t* ro
  * floor so CPU and GPU agree bit for bit: n <= 32 -> n Bernoulli(q) trials on
  * 32-bit uniforms; otherwise round(n q + sqrt(n q (1-q)) z) clamped to [0, n],
  * z an Irwin-Hall(12) normal (sum of 12 uniforms - 6).  Totals are exact. */
-extern "C" uint64_t oracle_binomial(uint64_t n, double q, uint64_t seed, uint64_t node) {
+static uint64_t binomial_purpose(uint64_t n, double q, uint64_t seed, uint64_t node, uint32_t purpose) {
     if (n == 0) return 0;
     if (n <= 32) {
         uint64_t cnt = 0;
         uint32_t x[4];
         for (uint64_t t = 0; t < n; ++t) {
-            if ((t & 3) == 0) philox_u64(node, ORACLE_PURPOSE_SECTION_COUNTS, (uint32_t)(t >> 2), seed, x);
+            if ((t & 3) == 0) philox_u64(node, purpose, (uint32_t)(t >> 2), seed, x);
             const double u = (double)x[t & 3] * 0x1.0p-32;
             if (u < q) ++cnt;
         }
@@ -554,7 +554,7 @@ extern "C" uint64_t oracle_binomial(uint64_t n, double q, uint64_t seed, uint64_
     double z = 0.0;
     for (uint32_t d = 0; d < 3; ++d) {
         uint32_t x[4];
-        philox_u64(node, ORACLE_PURPOSE_SECTION_COUNTS, d, seed, x);
+        philox_u64(node, purpose, d, seed, x);
         for (int k = 0; k < 4; ++k) z = z + (double)x[k] * 0x1.0p-32;
     }
     z = z - 6.0;
@@ -565,6 +565,10 @@ extern "C" uint64_t oracle_binomial(uint64_t n, double q, uint64_t seed, uint64_
     if (r < 0.0) r = 0.0;
     if (r > (double)n) r = (double)n;
     return (uint64_t)r;
+}
+
+extern "C" uint64_t oracle_binomial(uint64_t n, double q, uint64_t seed, uint64_t node) {
+    return binomial_purpose(n, q, seed, node, ORACLE_PURPOSE_SECTION_COUNTS);
 }
 
 static void counts_rec(int64_t lo, int64_t hi, uint64_t cnt, uint64_t node, int64_t n, int64_t SS,
@@ -924,4 +928,105 @@ extern "C" void oracle_set_threads(int32_t threads) {
 #else
     (void)threads;
 #endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sectioned rejection sampling (no counterpart in the reference; it combines
+ * SamplerRejection.cu:17-35 with the sections of SamplerGpuSectioned.cu:58-136).
+ * A is cut into sections of SS slots.  A trial on slot s accepts iff
+ * coin53 <= accept(s), accept(s) = frac(N w/W) on the first slot of an item's
+ * run and 1 elsewhere, so its acceptance probability is exactly
+ * P(s) = min(floor(accept(s) 2^53) + 1, 2^53) 2^-53.  An accepted sample lies
+ * in section k with probability M_k / sum M, M_k = sum of P(s) over the
+ * section, so the sample count of every section comes from the halving tree
+ * of the sectioned alias sampler with q = (Pre[mid] - Pre[lo]) /
+ * (Pre[hi] - Pre[lo]) on the canonical exclusive prefix Pre of the M_k; inside
+ * a section a sample repeats trials on uniform slots of the section.  M_k is
+ * the canonical section sum (lane-strided, pairwise tree). */
+static inline double rej_accept_prob(double accept) {
+    if (accept >= 1.0) return 1.0;
+    return (std::floor(accept * 0x1.0p53) + 1.0) * 0x1.0p-53;
+}
+
+static inline double rej_slot_accept(const int32_t* A, uint64_t s, const double* w, int64_t n, double W) {
+    const int32_t item = A[s];
+    if (s != 0 && A[s - 1] == item) return 1.0;
+    const double xv = ((double)n * w[item]) / W;
+    return xv - std::floor(xv);
+}
+
+/* Exclusive prefix Pre[0..nsec] (Pre[nsec] = total) of the section masses. */
+extern "C" void oracle_rejection_section_prefix(const int32_t* A, uint64_t A_size, const double* w, int64_t n,
+                                               double W, int64_t SS, double* pre) {
+    const int64_t nsec = (int64_t)((A_size + (uint64_t)SS - 1) / (uint64_t)SS);
+    std::vector<double> mass((size_t)nsec);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < nsec; ++k) {
+        const uint64_t a = (uint64_t)k * SS, b = std::min<uint64_t>(A_size, a + SS);
+        double acc[64];
+        for (int l = 0; l < 64; ++l) {
+            double x = 0.0;
+            for (uint64_t s = a + l; s < b; s += 64) x = x + rej_accept_prob(rej_slot_accept(A, s, w, n, W));
+            acc[l] = x;
+        }
+        mass[(size_t)k] = tree64(acc, 64);
+    }
+    pre[nsec] = oracle_local_scan(mass.data(), nsec, pre);
+}
+
+static void rej_counts_rec(int64_t lo, int64_t hi, uint64_t cnt, uint64_t node, const double* pre, uint64_t seed,
+                           uint64_t* counts) {
+    if (hi - lo == 1) { counts[lo] = cnt; return; }
+    const int64_t mid = lo + (hi - lo) / 2;
+    /* a range whose mass vanishes in the prefix's rounding splits by section count */
+    const double den = pre[hi] - pre[lo];
+    const double q = den > 0.0 ? (pre[mid] - pre[lo]) / den : (double)(mid - lo) / (double)(hi - lo);
+    const uint64_t x = binomial_purpose(cnt, q, seed, node, ORACLE_PURPOSE_REJ_SECTION_COUNTS);
+    rej_counts_rec(lo, mid, x, 2 * node, pre, seed, counts);
+    rej_counts_rec(mid, hi, cnt - x, 2 * node + 1, pre, seed, counts);
+}
+
+extern "C" void oracle_rejection_section_counts(const double* pre, int64_t nsec, uint64_t total, uint64_t seed,
+                                               uint64_t* counts) {
+    rej_counts_rec(0, nsec, total, 1, pre, seed, counts);
+}
+
+/* Samples of the sectioned rejection sampler: section k draws counts[k]
+ * samples written contiguously in section order; sample g makes trials
+ * t = 0, 1, ... with Philox counter (g, purpose, t), a uniform slot of the
+ * section from words 0-1 and coin53 from words 2-3, until one accepts (at
+ * most ORACLE_REJ_SECTIONED_MAX_TRIALS; then the last trial's item).
+ * Only the sections listed (all when sections == NULL) are drawn; their
+ * samples land at their output positions of the full run. */
+extern "C" void oracle_sample_rejection_sectioned(const int32_t* A, uint64_t A_size, const double* w, int64_t n,
+                                                 double W, int64_t SS, uint64_t total, uint64_t seed,
+                                                 const int64_t* sections, int64_t nsel, int32_t* out,
+                                                 uint64_t* first_out, uint64_t* cnt_out) {
+    const int64_t nsec = (int64_t)((A_size + (uint64_t)SS - 1) / (uint64_t)SS);
+    std::vector<double> pre((size_t)nsec + 1);
+    oracle_rejection_section_prefix(A, A_size, w, n, W, SS, pre.data());
+    std::vector<uint64_t> counts((size_t)nsec);
+    oracle_rejection_section_counts(pre.data(), nsec, total, seed, counts.data());
+    std::vector<uint64_t> off((size_t)nsec + 1, 0);
+    for (int64_t k = 0; k < nsec; ++k) off[(size_t)k + 1] = off[(size_t)k] + counts[(size_t)k];
+    const int64_t cnt = sections ? nsel : nsec;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < cnt; ++q) {
+        const int64_t k = sections ? sections[q] : q;
+        if (first_out) first_out[q] = off[(size_t)k];
+        if (cnt_out) cnt_out[q] = counts[(size_t)k];
+        const uint64_t a = (uint64_t)k * SS, len = std::min<uint64_t>(A_size, a + SS) - a;
+        for (uint64_t t = 0; t < counts[(size_t)k]; ++t) {
+            const uint64_t g = off[(size_t)k] + t;
+            int32_t item = 0;
+            for (uint32_t tr = 0; tr < ORACLE_REJ_SECTIONED_MAX_TRIALS; ++tr) {
+                uint32_t x[4];
+                philox_u64(g, ORACLE_PURPOSE_REJ_SECTIONED, tr, seed, x);
+                const uint64_t s = a + mulhi64(word64(x), len);
+                item = A[s];
+                if (!(coin53(x) > rej_slot_accept(A, s, w, n, W))) break;
+            }
+            out[g] = item;
+        }
+    }
 }

@@ -46,6 +46,9 @@ extern "C" {
 #define ORACLE_PURPOSE_SECTION_COUNTS 0x0000B003u
 #define ORACLE_PURPOSE_REJECTION 0x0000B004u
 #define ORACLE_PURPOSE_BATCH 0x0000B005u
+#define ORACLE_PURPOSE_REJ_SECTIONED 0x0000B006u
+#define ORACLE_PURPOSE_REJ_SECTION_COUNTS 0x0000B007u
+#define ORACLE_REJ_SECTIONED_MAX_TRIALS (1u << 20)
 #define ORACLE_REJECTION_MAX_TRIALS 256
 
 typedef struct {
@@ -147,6 +150,15 @@ void oracle_sample_sectioned_subset(const double* row_w, const int32_t* row_alia
                                     const int64_t* sections, int64_t nsel, int32_t* out, uint64_t* first,
                                     uint64_t* cnt);
 void oracle_set_threads(int32_t threads);
+/* Sectioned rejection sampling (see atg_oracle.cpp). */
+void oracle_rejection_section_prefix(const int32_t* A, uint64_t A_size, const double* w, int64_t n, double W,
+                                     int64_t section_slots, double* prefix /* nsec + 1 */);
+void oracle_rejection_section_counts(const double* prefix, int64_t nsec, uint64_t total, uint64_t seed,
+                                     uint64_t* counts);
+void oracle_sample_rejection_sectioned(const int32_t* A, uint64_t A_size, const double* w, int64_t n, double W,
+                                       int64_t section_slots, uint64_t total, uint64_t seed,
+                                       const int64_t* sections /* nullable: all */, int64_t nsel,
+                                       int32_t* out /* total */, uint64_t* first_out, uint64_t* cnt_out);
 
 #ifdef __cplusplus
 }

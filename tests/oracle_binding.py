@@ -295,3 +295,44 @@ def sample_sectioned_subset(row_w, row_alias, W_N, section_rows, total, seed, se
 
 def set_threads(threads):
     L.oracle_set_threads(int(threads))
+
+
+L.oracle_rejection_section_prefix.argtypes = [P, C.c_uint64, P, C.c_int64, C.c_double, C.c_int64, P]
+L.oracle_rejection_section_counts.argtypes = [P, C.c_int64, C.c_uint64, C.c_uint64, P]
+L.oracle_sample_rejection_sectioned.argtypes = [P, C.c_uint64, P, C.c_int64, C.c_double, C.c_int64, C.c_uint64,
+                                                C.c_uint64, P, C.c_int64, P, P, P]
+
+
+def rejection_section_prefix(A, w, W, section_slots):
+    """Canonical exclusive prefix (nsec + 1 values) of the sections'
+    acceptance masses."""
+    A = np.ascontiguousarray(A, dtype=np.int32)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    nsec = (len(A) + section_slots - 1) // section_slots
+    pre = np.empty(nsec + 1)
+    L.oracle_rejection_section_prefix(ptr(A), len(A), ptr(w), len(w), float(W), int(section_slots), ptr(pre))
+    return pre
+
+
+def rejection_section_counts(pre, total, seed):
+    pre = np.ascontiguousarray(pre, dtype=np.float64)
+    c = np.empty(len(pre) - 1, np.uint64)
+    L.oracle_rejection_section_counts(ptr(pre), len(pre) - 1, int(total), int(seed), ptr(c))
+    return c
+
+
+def sample_rejection_sectioned(A, w, W, section_slots, total, seed, sections=None):
+    """Sectioned rejection samples: all of them (sections None), or the
+    samples of the given sections at their positions of the full run (other
+    positions untouched), with each given section's first position and count."""
+    A = np.ascontiguousarray(A, dtype=np.int32)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.empty(int(total), np.int32)
+    sel = None if sections is None else np.ascontiguousarray(sections, dtype=np.int64)
+    k = 0 if sel is None else len(sel)
+    first = np.empty(max(k, 1), np.uint64)
+    cnt = np.empty(max(k, 1), np.uint64)
+    L.oracle_sample_rejection_sectioned(ptr(A), len(A), ptr(w), len(w), float(W), int(section_slots), int(total),
+                                        int(seed), ptr(sel), k, ptr(out), ptr(first) if sel is not None else None,
+                                        ptr(cnt) if sel is not None else None)
+    return (out, first[:k], cnt[:k]) if sel is not None else out

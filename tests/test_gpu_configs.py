@@ -146,3 +146,17 @@ def test_rejection_and_basic_headline(gpu, oracle, headline):
     t.sample(count, seed=4, kind="basic", out=out)
     exp = oracle.sample_basic(ref["row_w"], ref["row_alias"], ref["stats"]["W_N"], 4, count)
     np.testing.assert_array_equal(out.cpu().numpy(), exp)
+    # 1e9 samples by sections of A, a few hundred whole sections (the first
+    # and the last among them) against the oracle
+    del out
+    S, slots = 1_000_000_000, 4096
+    out = torch.empty(S, dtype=torch.int32, device="cuda")
+    t.rejection_sample_sectioned(S, seed=21, section_slots=slots, out=out)
+    torch.cuda.synchronize()
+    nsec = (A_ref.size + slots - 1) // slots
+    sel = np.unique(np.concatenate([[0, nsec - 1], np.random.default_rng(2).integers(0, nsec, 300)]))
+    exp, first, cnt = oracle.sample_rejection_sectioned(A_ref, w, W, slots, S, 21, sections=sel)
+    assert int(first[-1] + cnt[-1]) == S
+    for s, f, c in zip(sel.tolist(), first.tolist(), cnt.tolist()):
+        assert np.array_equal(out[f:f + c].cpu().numpy(), exp[f:f + c]), f"section {s}"
+    log(f"sectioned rejection: {sel.size} sections, {int(cnt.sum())} samples compared")

@@ -150,3 +150,40 @@ def test_basic_decision_boundaries(gpu, oracle):
     for k, exp in expected.items():
         assert got[k] == exp, (k, k % 5, got[k], exp)
     np.testing.assert_array_equal(got, oracle.sample_basic(rw, ra, 1.0, seed, count))
+
+
+@pytest.mark.parametrize("n,dist,slots,count", [(100003, "pareto", 4096, 3_000_000), (1 << 20, "exponential", 4096, 4_000_000),
+                                                (50000, "one_heavy", 1024, 1_000_000), (4097, "powerlaw2", 8192, 500_000),
+                                                (300001, "heavy_mix", 2048, 2_000_000), (1, "unit_uniform", 4096, 1000)])
+def test_rejection_sectioned_matches_oracle(gpu, oracle, n, dist, slots, count):
+    """Rejection sampling by LDS-staged sections of A: the oracle's items on
+    the same Philox stream (section masses, prefix, counts and in-section
+    trials all exact), twice on one structure (cached masses), then with
+    another section size."""
+    torch = gpu
+    w = atg.generate_weights(dist, n, seed=5, param=1.1 if dist == "pareto" else 0.9)
+    t = atg.Table(n)
+    t.set_weights(w)
+    t.rejection_build()
+    W = t.rejection_weight_sum()
+    _, A = oracle.rejection_build(w, W)
+    out = torch.empty(count, dtype=torch.int32, device="cuda")
+    for seed, ss in ((11, slots), (12, slots), (13, 512)):
+        cs = t.rejection_sample_sectioned(count, seed=seed, section_slots=ss, out=out, checksum=True)
+        exp = oracle.sample_rejection_sectioned(A, w, W, ss, count, seed)
+        np.testing.assert_array_equal(out.cpu().numpy(), exp, err_msg=f"section_slots={ss} seed={seed}")
+        assert cs == int(exp.astype(np.int64).sum())
+
+
+def test_rejection_sectioned_chi_squared(gpu):
+    torch = gpu
+    n = 20000
+    w = atg.generate_weights("sine", n)
+    t = atg.Table(n)
+    t.set_weights(w)
+    t.rejection_build()
+    hist = torch.zeros(n, dtype=torch.int32, device="cuda")
+    t.rejection_sample_sectioned(20_000_000, seed=6, section_slots=1024, hist=hist)
+    h = hist.cpu().numpy().astype(np.float64)
+    assert h.sum() == 20_000_000
+    assert chi2_pvalue(h, w) > 1e-4

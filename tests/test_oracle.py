@@ -469,3 +469,37 @@ def test_sectioned_subset_equals_full_run(oracle):
         assert np.array_equal(out[pos:pos + c], full[f:f + c])
         pos += c
     assert pos == len(out)
+
+
+# ---- sectioned rejection sampling ------------------------------------------
+
+def test_rejection_sectioned_masses_and_distribution(oracle):
+    n = 20000
+    w = atg.generate_weights("pareto", n, seed=9, param=1.1)
+    W = oracle.weight_sum(w, 1000)[0]
+    incl, A = oracle.rejection_build(w, W)
+    SS = 1024
+    pre = oracle.rejection_section_prefix(A, w, W, SS)
+    # The acceptance masses add up to N (sum over items of N w_i / W) up to
+    # the 2^-53 coin granularity and rounding.
+    assert abs(pre[-1] - n) < 1e-9 * n
+    assert np.all(np.diff(pre) > 0)
+    # Section masses against a direct numpy statement: first slot of a run
+    # accepts with frac(N w / W), the others always.
+    first = np.ones(len(A), bool)
+    first[1:] = A[1:] != A[:-1]
+    xv = (n * w[A]) / W
+    acc = np.where(first, xv - np.floor(xv), 1.0)
+    prob = np.where(acc >= 1.0, 1.0, (np.floor(acc * 2.0**53) + 1.0) * 2.0**-53)
+    masses = np.add.reduceat(prob, np.arange(0, len(A), SS))
+    np.testing.assert_allclose(np.diff(pre), masses, rtol=1e-12)
+    # Exact totals and a chi-squared fit of the samples to the weights.
+    c = oracle.rejection_section_counts(pre, 3_000_000, 5)
+    assert int(c.sum()) == 3_000_000
+    out = oracle.sample_rejection_sectioned(A, w, W, SS, 2_000_000, 7)
+    assert _chi2(np.bincount(out, minlength=n).astype(float), w) > 1e-4
+    # A subset of sections reproduces the full run at the same positions.
+    sel = [0, 3, len(pre) - 2]
+    sub, f, k = oracle.sample_rejection_sectioned(A, w, W, SS, 2_000_000, 7, sections=sel)
+    for a, b in zip(f.tolist(), k.tolist()):
+        assert np.array_equal(sub[a:a + b], out[a:a + b])
