@@ -57,6 +57,16 @@ def _device_buffer(x, kind, numel, name, exact=False):
     return x
 
 
+def _split_id(split):
+    if split not in _lib.SPLIT_METHODS:
+        raise AtgError(_EINVAL, f"unknown split method {split!r}; one of {sorted(_lib.SPLIT_METHODS)}")
+    return _lib.SPLIT_METHODS[split]
+
+
+def _split_name(method):
+    return {v: k for k, v in _lib.SPLIT_METHODS.items()}[int(method)]
+
+
 def _stream(s):
     if s is None:
         return None
@@ -106,7 +116,7 @@ def _shape_opts(psa_plus, chains, section_size, num_splits, split, learned_segme
     o.chains = int(chains)
     o.section_size = int(section_size)
     o.num_splits = int(num_splits)
-    o.split_method = ATG_SPLIT_LEARNED if split == "learned" else ATG_SPLIT_BISECT
+    o.split_method = _split_id(split)
     o.learned_segments = int(learned_segments)
     return o
 
@@ -123,7 +133,7 @@ def autotune(w, dtype="f64", layout="aos", psa_plus=True, chains=16, section_siz
     check(lib().atg_autotune(ptr, mem, int(n), ATG_F32 if dtype == "f32" else ATG_F64,
                              ATG_SOA if layout == "soa" else ATG_AOS, C.byref(o), int(reps), C.byref(best),
                              C.byref(ms)))
-    return {"split": "learned" if best.split_method == ATG_SPLIT_LEARNED else "bisect",
+    return {"split": _split_name(best.split_method),
             "learned_segments": int(best.learned_segments), "interleave": bool(best.interleave)}, ms.value
 
 
@@ -203,7 +213,7 @@ class Table:
         o.chains = int(chains)
         o.section_size = int(section_size)
         o.num_splits = int(num_splits)
-        o.split_method = ATG_SPLIT_LEARNED if split == "learned" else ATG_SPLIT_BISECT
+        o.split_method = _split_id(split)
         o.learned_segments = int(learned_segments)
         o.record_phases = 1 if record_phases else 0
         self.n = int(n)
@@ -378,7 +388,7 @@ class Batch:
         o.chains = int(chains)
         o.section_size = int(section_size)
         o.num_splits = int(num_splits)
-        o.split_method = ATG_SPLIT_LEARNED if split == "learned" else ATG_SPLIT_BISECT
+        o.split_method = _split_id(split)
         o.learned_segments = int(learned_segments)
         self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         self.ntables = self.offsets.size - 1
@@ -463,7 +473,7 @@ class Batch:
         ms, p99 = C.c_double(0.0), C.c_double(0.0)
         check(lib().atg_batch_autotune(self.h, int(reps), 1 if tune_split else 0, float(makespan_slack), C.byref(o),
                                        C.byref(ms), C.byref(p99)))
-        return {"split": "learned" if o.split_method == ATG_SPLIT_LEARNED else "bisect",
+        return {"split": _split_name(o.split_method),
                 "group_items": int(o.batch_group_items), "makespan_ms": ms.value, "p99_latency_ms": p99.value}
 
     def latencies_ns(self):

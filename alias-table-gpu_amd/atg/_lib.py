@@ -15,7 +15,8 @@ STATUS_NAMES = {0: "ATG_OK", 1: "ATG_EINVAL", 2: "ATG_ENOMEM", 3: "ATG_EDEVICE",
 ATG_F64, ATG_F32 = 0, 1
 ATG_AOS, ATG_SOA = 0, 1
 ATG_MEM_HOST, ATG_MEM_DEVICE = 0, 1
-ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED = 0, 1
+ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED, ATG_SPLIT_PARY = 0, 1, 2
+SPLIT_METHODS = {"bisect": ATG_SPLIT_BISECT, "learned": ATG_SPLIT_LEARNED, "pary": ATG_SPLIT_PARY}
 ATG_SAMPLE_BASIC, ATG_SAMPLE_SECTIONED = 0, 1
 
 DIST = {

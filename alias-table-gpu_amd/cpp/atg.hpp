@@ -92,7 +92,9 @@ class AliasTableSplitGpu : public SamplingAlgorithm {
         if (table_) atg_destroy(table_);
     }
     std::string name() override {
-        std::string v = variant_.split == ATG_SPLIT_LEARNED ? "LearnedSplit" : "";
+        std::string v = variant_.split == ATG_SPLIT_LEARNED ? "LearnedSplit"
+                        : variant_.split == ATG_SPLIT_PARY  ? "ParySplit"
+                                                            : "";
         if (variant_.interleave) v += "Interleaved";
         if (variant_.psaPlus) v += "PsaPlus";
         return std::string("AliasTableSplitGpu") +

@@ -65,7 +65,8 @@ static void testRandomRounds(uint32_t seed) {
     std::mt19937 rng(seed);
     struct V { atg_split_method split; bool interleave; bool psa_ok; };
     const V variants[] = {{ATG_SPLIT_BISECT, false, true}, {ATG_SPLIT_LEARNED, false, true},
-                          {ATG_SPLIT_BISECT, true, true}, {ATG_SPLIT_LEARNED, true, true}};
+                          {ATG_SPLIT_BISECT, true, true}, {ATG_SPLIT_LEARNED, true, true},
+                          {ATG_SPLIT_PARY, false, true}, {ATG_SPLIT_PARY, true, true}};
     const int numTests = 20;
     std::printf("random rounds (seed %u)\n", seed);
     for (int i = 0; i < numTests; ++i) {

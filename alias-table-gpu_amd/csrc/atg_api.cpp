@@ -127,7 +127,7 @@ struct atg_table_s {
         a.tree_scratch_half = tree_half;
         a.slots = d_slots;
         a.sec_cert = d_sec_cert;
-        a.want_cert = opts.split_method == ATG_SPLIT_LEARNED;
+        a.want_cert = opts.split_method != ATG_SPLIT_BISECT;  // the learned and P-ary searches need it
         a.stage_w = d_stage_w;
         a.stage_i = d_stage_i;
         a.Li = d_Li; a.Lw = d_Lw; a.Lp = d_Lp;
@@ -201,7 +201,7 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     atg_build_opts o;
     if (opts) o = *opts; else atg_default_opts(&o);
     if (o.chains < 1 || o.chains > 64) return fail(ATG_EINVAL, "atg_create: chains=%d outside [1, 64]", o.chains);
-    if (o.split_method != ATG_SPLIT_BISECT && o.split_method != ATG_SPLIT_LEARNED)
+    if (o.split_method != ATG_SPLIT_BISECT && o.split_method != ATG_SPLIT_LEARNED && o.split_method != ATG_SPLIT_PARY)
         return fail(ATG_EINVAL, "atg_create: unknown split method %d", o.split_method);
     if (o.learned_segments <= 0) o.learned_segments = 4096;
     if (o.section_size <= 0) o.section_size = std::min<int64_t>(1000, n / 10);
@@ -323,6 +323,7 @@ atg_status atg_build_phase(atg_table* t, int32_t phase, void* stream) {
         case ATG_PHASE_PARTITION_PLACE: HIPCHK(launch_partition_place(a, s)); break;
         case ATG_PHASE_SPLIT:
             if (t->opts.split_method == ATG_SPLIT_LEARNED) HIPCHK(launch_split_learned(a, s));
+            else if (t->opts.split_method == ATG_SPLIT_PARY) HIPCHK(launch_split_pary(a, s));
             else HIPCHK(launch_split(a, s));
             break;
         case ATG_PHASE_PACK:
@@ -343,7 +344,7 @@ static bool build_split_pack_interleaved(atg_table* t, const BuildArgs& a, hipSt
     const int64_t SB = split_blocks(a), PB = pack_blocks(a);
     const int C = (int)std::min<int64_t>(kInterleaveChunks, SB);
     if (C < 2) return false;
-    const bool learned = t->opts.split_method == ATG_SPLIT_LEARNED;
+    const bool learned = t->opts.split_method == ATG_SPLIT_LEARNED, pary = t->opts.split_method == ATG_SPLIT_PARY;
     hipStream_t sp = t->sp_stream;
     err = hipEventRecord(t->ev_fork, s);
     if (err == hipSuccess) err = hipStreamWaitEvent(sp, t->ev_fork, 0);
@@ -351,7 +352,9 @@ static bool build_split_pack_interleaved(atg_table* t, const BuildArgs& a, hipSt
     if (err == hipSuccess && learned) err = launch_split_knots(a, sp);
     for (int c = 0; c < C && err == hipSuccess; ++c) {
         const int64_t b0 = SB * c / C, b1 = SB * (c + 1) / C;
-        err = learned ? launch_split_learned_blocks(a, sp, b0, b1) : launch_split_blocks(a, sp, b0, b1);
+        err = learned ? launch_split_learned_blocks(a, sp, b0, b1)
+              : pary  ? launch_split_pary_blocks(a, sp, b0, b1)
+                      : launch_split_blocks(a, sp, b0, b1);
         if (err == hipSuccess) err = hipEventRecord(t->ev_chunk[c], sp);
     }
     for (int c = 0; c < C && err == hipSuccess; ++c) {
@@ -889,7 +892,7 @@ atg_status table_set_shape(atg_table* t, int32_t chains, int64_t num_splits) {
 // interleave (its second stream is created on first use).
 atg_status table_set_method(atg_table* t, int32_t split_method, int32_t learned_segments, int32_t interleave) {
     if (!t) return fail(ATG_EINVAL, "table_set_method: NULL table");
-    if (split_method != ATG_SPLIT_BISECT && split_method != ATG_SPLIT_LEARNED)
+    if (split_method != ATG_SPLIT_BISECT && split_method != ATG_SPLIT_LEARNED && split_method != ATG_SPLIT_PARY)
         return fail(ATG_EINVAL, "table_set_method: unknown split method %d", split_method);
     if (learned_segments < 1 || learned_segments > t->knots_cap)
         return fail(ATG_EINVAL, "table_set_method: %d knots outside [1, %d]", learned_segments, t->knots_cap);

@@ -458,6 +458,8 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
     atg_build_opts o;
     if (opts) o = *opts; else atg_default_opts(&o);
     if (o.chains < 1 || o.chains > 64) return fail(ATG_EINVAL, "atg_batch_create: chains=%d outside [1, 64]", o.chains);
+    if (o.split_method == ATG_SPLIT_PARY)
+        return fail(ATG_ENOTSUP, "atg_batch_create: the P-ary split is built for single tables; use bisection or the learned split");
     if (o.split_method != ATG_SPLIT_BISECT && o.split_method != ATG_SPLIT_LEARNED)
         return fail(ATG_EINVAL, "atg_batch_create: unknown split method %d", o.split_method);
     if (o.learned_segments <= 0) o.learned_segments = 4096;

@@ -83,6 +83,7 @@ hipError_t launch_partition_place(const BuildArgs& a, hipStream_t s);
 hipError_t launch_degenerate(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split_learned(const BuildArgs& a, hipStream_t s);
+hipError_t launch_split_pary(const BuildArgs& a, hipStream_t s);
 hipError_t launch_pack(const BuildArgs& a, hipStream_t s);
 // Chunked split and pack (split/pack interleave): blocks [b0, b1) of
 // split_blocks (256 splits each) or pack_blocks (64 chains each).
@@ -91,6 +92,7 @@ int64_t pack_blocks(const BuildArgs& a);
 hipError_t launch_split_blocks(const BuildArgs& a, hipStream_t s, int64_t b0, int64_t b1);
 hipError_t launch_split_knots(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split_learned_blocks(const BuildArgs& a, hipStream_t s, int64_t b0, int64_t b1);
+hipError_t launch_split_pary_blocks(const BuildArgs& a, hipStream_t s, int64_t b0, int64_t b1);
 hipError_t launch_pack_blocks(const BuildArgs& a, hipStream_t s, int64_t b0, int64_t b1);
 hipError_t launch_split_rows(const Row* rows, double* w, int32_t* al, int64_t n, hipStream_t s);
 hipError_t launch_join_rows(const double* w, const int32_t* al, Row* rows, int64_t n, hipStream_t s);

@@ -87,7 +87,7 @@ extern "C" atg_status atg_build_noalloc(const void* weights, atg_dtype dtype, in
     if (o.interleave || o.record_phases)
         return fail(ATG_ENOTSUP, "atg_build_noalloc: interleave and record_phases need streams and events of their own");
     if (o.chains < 1 || o.chains > 64) return fail(ATG_EINVAL, "atg_build_noalloc: chains=%d outside [1, 64]", o.chains);
-    if (o.split_method != ATG_SPLIT_BISECT && o.split_method != ATG_SPLIT_LEARNED)
+    if (o.split_method != ATG_SPLIT_BISECT && o.split_method != ATG_SPLIT_LEARNED && o.split_method != ATG_SPLIT_PARY)
         return fail(ATG_EINVAL, "atg_build_noalloc: unknown split method %d", o.split_method);
     // The shape exactly as atg_create derives it.
     if (o.learned_segments <= 0) o.learned_segments = 4096;
@@ -120,7 +120,7 @@ extern "C" atg_status atg_build_noalloc(const void* weights, atg_dtype dtype, in
     a.tree_scratch_half = tree_half;
     a.slots = c.take<uint64_t>(3 * tree_total + (tree_total + 1) / 2 + 1);
     a.sec_cert = c.take<uint64_t>(6 * G);
-    a.want_cert = o.split_method == ATG_SPLIT_LEARNED;
+    a.want_cert = o.split_method != ATG_SPLIT_BISECT;
     a.stage_w = c.take<double>(n);
     a.stage_i = c.take<int32_t>(n);
     a.Li = c.take<int32_t>(n + 2);
@@ -153,7 +153,9 @@ extern "C" atg_status atg_build_noalloc(const void* weights, atg_dtype dtype, in
     HIPCHK(hipGetLastError());
     HIPCHK(launch_sum(a, s));
     HIPCHK(launch_partition(a, s));
-    HIPCHK(o.split_method == ATG_SPLIT_LEARNED ? launch_split_learned(a, s) : launch_split(a, s));
+    HIPCHK(o.split_method == ATG_SPLIT_LEARNED ? launch_split_learned(a, s)
+           : o.split_method == ATG_SPLIT_PARY  ? launch_split_pary(a, s)
+                                               : launch_split(a, s));
     HIPCHK(launch_pack(a, s));
     const char* st = reinterpret_cast<const char*>(a.st);
     if (W_out) HIPCHK(hipMemcpyAsync(W_out, st + offsetof(DevState, W), sizeof(double), hipMemcpyDeviceToDevice, s));

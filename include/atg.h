@@ -40,7 +40,9 @@ typedef enum { ATG_MEM_HOST = 0, ATG_MEM_DEVICE = 1 } atg_mem;
 
 typedef enum {
     ATG_SPLIT_BISECT = 0,   /* SplitMethodBasic (SplitMethodBasic.cu:14-50) */
-    ATG_SPLIT_LEARNED = 1   /* piecewise-linear index + exact local search, bisection fallback */
+    ATG_SPLIT_LEARNED = 1,  /* piecewise-linear index + exact local search, bisection fallback */
+    ATG_SPLIT_PARY = 2      /* SplitMethodParySearch (SplitMethodParySearch.cu:9-145): 16 lanes per split
+                               probe 16 positions per step; bisection fallback; single tables only */
 } atg_split_method;
 
 typedef enum {

@@ -1030,3 +1030,31 @@ extern "C" void oracle_sample_rejection_sectioned(const int32_t* A, uint64_t A_s
         }
     }
 }
+
+/* SplitMethodInverse (SplitMethodInverse.cu:3-31), a test-only second
+ * brute force of the split: every heavy position j visits every split k
+ * with j < n'(k) and records (i, j, spill) wherever the FOUND predicate
+ * holds (sequentially, so the largest such j of a k is kept; the reference
+ * runs the j in parallel).  Inputs as oracle_split_one; splits[0] and
+ * splits[p] are the fixed ends; a k that no j satisfies keeps i = j = -1. */
+extern "C" void oracle_split_inverse(int64_t p, int64_t nToSplit, double wToSplit, const double* Lp, int64_t numL,
+                                    const double* Hp, const double* Hw, int64_t numH, int32_t* si, int32_t* sj,
+                                    double* sspill) {
+    for (int64_t k = 0; k <= p; ++k) { si[k] = -1; sj[k] = -1; sspill[k] = 0.0; }
+    si[0] = 0; sj[0] = 0;
+    si[p] = (int32_t)(numL - 1); sj[p] = (int32_t)(numH - 1);
+    for (int64_t j = 0; j < numH - 1; ++j) {
+        for (int64_t k = 1; k < p; ++k) {
+            const int64_t n_ = (int64_t)std::ceil(((double)nToSplit * (double)k) / (double)p);
+            if (j >= n_) continue;
+            const int64_t i = std::min(n_ - j, numL - 1);
+            const double target = (wToSplit * (double)n_) / (double)nToSplit;
+            if (i < 1) continue;
+            const double sigma = Lp[i] + Hp[j];
+            const double sigmaNext = Lp[i - 1] + Hp[j + 1];
+            if (sigma <= target && sigmaNext > target) {
+                si[k] = (int32_t)i; sj[k] = (int32_t)j; sspill[k] = sigma + Hw[j] - target;
+            }
+        }
+    }
+}

@@ -150,6 +150,10 @@ void oracle_sample_sectioned_subset(const double* row_w, const int32_t* row_alia
                                     const int64_t* sections, int64_t nsel, int32_t* out, uint64_t* first,
                                     uint64_t* cnt);
 void oracle_set_threads(int32_t threads);
+/* SplitMethodInverse (test-only second brute force of the split). */
+void oracle_split_inverse(int64_t p, int64_t nToSplit, double wToSplit, const double* Lp, int64_t numL,
+                          const double* Hp, const double* Hw, int64_t numH, int32_t* split_i, int32_t* split_j,
+                          double* split_spill);
 /* Sectioned rejection sampling (see atg_oracle.cpp). */
 void oracle_rejection_section_prefix(const int32_t* A, uint64_t A_size, const double* w, int64_t n, double W,
                                      int64_t section_slots, double* prefix /* nsec + 1 */);

@@ -336,3 +336,18 @@ def sample_rejection_sectioned(A, w, W, section_slots, total, seed, sections=Non
                                         int(seed), ptr(sel), k, ptr(out), ptr(first) if sel is not None else None,
                                         ptr(cnt) if sel is not None else None)
     return (out, first[:k], cnt[:k]) if sel is not None else out
+
+
+L.oracle_split_inverse.argtypes = [C.c_int64, C.c_int64, C.c_double, P, C.c_int64, P, P, C.c_int64, P, P, P]
+
+
+def split_inverse(p, nToSplit, wToSplit, Lp, Hp, Hw):
+    """Splits by SplitMethodInverse's brute force (every heavy position
+    against every split); i = j = -1 where no position satisfies FOUND."""
+    si = np.empty(p + 1, np.int32)
+    sj = np.empty(p + 1, np.int32)
+    ss = np.empty(p + 1)
+    Lp, Hp, Hw = (np.ascontiguousarray(x, dtype=np.float64) for x in (Lp, Hp, Hw))
+    L.oracle_split_inverse(int(p), int(nToSplit), float(wToSplit), ptr(Lp), len(Lp), ptr(Hp), ptr(Hw), len(Hp),
+                           ptr(si), ptr(sj), ptr(ss))
+    return si, sj, ss

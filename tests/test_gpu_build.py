@@ -78,14 +78,16 @@ def test_build_matches_oracle(gpu, oracle, n, dist, psa, chains, section, p):
     assert_same_build(t, ref, n)
 
 
+@pytest.mark.parametrize("split", ["learned", "pary"])
 @pytest.mark.parametrize("n,dist,p,segments", [
     (1 << 20, "pareto", 65536, 4096), (1 << 20, "heavy_mix", 100000, 4096), (3_000_000, "unit_uniform", 300000, 1024),
     (100003, "sine", 10000, 4096), (100003, "pareto", 7, 4096), (2_000_003, "zipf_shuffled", 200000, 4096)])
-def test_learned_split_bit_identical(gpu, oracle, n, dist, p, segments):
-    """The learned index returns the bisection's splits: identical tables."""
+def test_learned_split_bit_identical(gpu, oracle, n, dist, p, segments, split):
+    """The learned index and the P-ary search return the bisection's splits:
+    identical tables."""
     w = atg.generate_weights(dist, n, seed=5, param=1.1 if dist == "pareto" else 0.0)
     plain = gpu_build(w, num_splits=p)
-    learned = gpu_build(w, num_splits=p, split="learned", learned_segments=segments)
+    learned = gpu_build(w, num_splits=p, split=split, learned_segments=segments)
     ref = oracle.build(w, num_splits=p)
     assert_same_build(learned, ref, n)
     for a, b in zip(plain.splits(), learned.splits()):
@@ -210,6 +212,7 @@ def test_split_ratio_mode(gpu, oracle):
 @pytest.mark.parametrize("n,dist,split,layout,p", [
     (1 << 20, "pareto", "bisect", "aos", 1 << 14),
     (1 << 20, "heavy_mix", "learned", "aos", 1 << 14),
+    (1 << 20, "pareto", "pary", "aos", 1 << 14),
     (300_000, "exponential", "bisect", "soa", 0),
     (5000, "pareto", "bisect", "aos", 0),  # too few splits to chunk: sequential
 ])

@@ -503,3 +503,26 @@ def test_rejection_sectioned_masses_and_distribution(oracle):
     sub, f, k = oracle.sample_rejection_sectioned(A, w, W, SS, 2_000_000, 7, sections=sel)
     for a, b in zip(f.tolist(), k.tolist()):
         assert np.array_equal(sub[a:a + b], out[a:a + b])
+
+
+def test_split_bisection_matches_inverse_split(oracle):
+    # SplitMethodInverse (SplitMethodInverse.cu:3-31) as a second brute force:
+    # where the split certificate holds (max light increment < min heavy
+    # increment: one FOUND position per split), its (i, j, spill) equal the
+    # bisection's bit for bit, for plain and PSA+ leftovers.
+    rng = np.random.default_rng(23)
+    checked = 0
+    for trial in range(60):
+        n = int(rng.integers(50, 3000))
+        dist = ["pareto", "exponential", "sine", "heavy_mix", "unit_uniform"][trial % 5]
+        w = atg.generate_weights(dist, n, seed=100 + trial, param=1.1 if dist == "pareto" else 0.9)
+        ref = oracle.build(w, psa_plus=bool(trial % 2), num_splits=int(rng.integers(2, 60)))
+        st = ref["stats"]
+        if st["p"] == 0 or st["monotone"] != 1:
+            continue
+        si, sj, ss = oracle.split_inverse(st["p"], st["nToSplit"], st["wToSplit"], ref["L_pref"], ref["H_pref"],
+                                          ref["H_w"])
+        assert np.array_equal(si, ref["split_i"]) and np.array_equal(sj, ref["split_j"]), trial
+        assert np.array_equal(ss.view(np.uint64), ref["split_spill"].view(np.uint64)), trial
+        checked += 1
+    assert checked >= 30

@@ -53,7 +53,7 @@ def check_against_build(torch, n, dist, layout, dtype, split, seed, **shape):
 
 
 @pytest.mark.parametrize("layout,dtype,split", [("aos", "f64", "bisect"), ("soa", "f32", "learned"),
-                                                ("aos", "f64", "learned")])
+                                                ("aos", "f64", "learned"), ("aos", "f64", "pary")])
 def test_noalloc_equals_build_two_calls(gpu, layout, dtype, split):
     torch = gpu
     # Two calls in a row: the second with a smaller input, so stale scratch
