@@ -199,7 +199,7 @@ def bench_learned(args, stream):
     res = {"workload": f"N={n:.0e} heavy_mix (1% heavy, 90% mass), {args.learned_segments} segments",
            "section_size": shape["section_size"], "chains": shape["chains"], "items_per_split": args.items_per_split}
     tabs = {}
-    for split in ("bisect", "learned"):
+    for split in ("bisect", "learned", "pary"):
         t = atg.Table(n, split=split, learned_segments=args.learned_segments, record_phases=True, **shape)
         t.set_weights(w, stream)
         for _ in range(args.warmup):
@@ -209,9 +209,10 @@ def bench_learned(args, stream):
         st = t.stats()
         res[split] = {"build_ms": round(ms, 4), "split_ms": round(st["phase_ms"][3], 4),
                       "gitems_s": round(n / (ms * 1e-3) / 1e9, 3), "p": st["p"]}
-        if split == "learned":
-            res[split]["learned_fallbacks"] = st["learned_fallbacks"]
+        if split != "bisect":
+            res[split]["search_fallbacks"] = st["learned_fallbacks"]
             res[split]["monotone"] = st["monotone"]
+        if split == "learned":
             rel, ok = t.post_build(verify=True)
             res["max_rel_err"] = rel
             res["reference_check_ok"] = ok
@@ -220,8 +221,10 @@ def bench_learned(args, stream):
         torch.cuda.empty_cache()
     res["speedup"] = round(res["bisect"]["build_ms"] / res["learned"]["build_ms"], 4)
     res["split_speedup"] = round(res["bisect"]["split_ms"] / max(res["learned"]["split_ms"], 1e-9), 3)
-    a, b = tabs["bisect"], tabs["learned"]
-    res["bit_identical"] = bool(np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64)) and np.array_equal(a[1], b[1]))
+    res["pary_speedup"] = round(res["bisect"]["build_ms"] / res["pary"]["build_ms"], 4)
+    a = tabs["bisect"]
+    res["bit_identical"] = all(bool(np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64)) and
+                                    np.array_equal(a[1], b[1])) for b in (tabs["learned"], tabs["pary"]))
     log(f"[bench] learned vs plain: {res}")
     return res
 
