@@ -335,8 +335,43 @@ class Table:
                                C.byref(cs) if checksum else None, _stream(stream)))
         return cs.value if checksum else None
 
-    def rejection_build(self, stream=None):
-        check(lib().atg_rejection_build(self.h, _stream(stream)))
+    def rejection_build(self, stream=None, fill="direct", knots=0):
+        """RejectionSampling::build with the fill of A chosen by name
+        (atg_rejection_build_fill): "direct", "bisect", "pary" or "learned"
+        (knots = the learned model's segments, 0 = 4096)."""
+        if fill not in _lib.FILL_METHODS:
+            raise AtgError(_EINVAL, f"rejection_build: unknown fill {fill!r}, expected one of {sorted(_lib.FILL_METHODS)}")
+        check(lib().atg_rejection_build_fill(self.h, _lib.FILL_METHODS[fill], int(knots), _stream(stream)))
+
+    def rejection_fill(self, stream=None):
+        """RejectionSampling::fillA alone: the last build's fill again (atg_rejection_fill_a)."""
+        check(lib().atg_rejection_fill_a(self.h, _stream(stream)))
+
+    def rejection_fill_fallbacks(self):
+        v = C.c_uint64(0)
+        check(lib().atg_rejection_fill_fallbacks(self.h, C.byref(v)))
+        return v.value
+
+    def rejection_model(self, knots):
+        """The learned fill's model: knot items (knots + 1) and error bounds (knots)."""
+        q = np.empty(int(knots) + 1, np.int32)
+        e = np.empty(int(knots), np.uint32)
+        check(lib().atg_rejection_get_model(self.h, _ptr(q), _ptr(e), int(knots)))
+        return q, e
+
+    def rejection_set_model(self, knot_items, err_bounds):
+        q = np.ascontiguousarray(knot_items, dtype=np.int32)
+        e = np.ascontiguousarray(err_bounds, dtype=np.uint32)
+        if len(q) != len(e) + 1:
+            raise AtgError(_EINVAL, f"rejection_set_model: {len(q)} knot items for {len(e)} bounds")
+        check(lib().atg_rejection_set_model(self.h, _ptr(q), _ptr(e), len(e)))
+
+    def rejection_accept(self):
+        """The acceptance value of every slot of A."""
+        sz = self.rejection_size()
+        a = np.empty(sz, np.float64)
+        check(lib().atg_get_rejection_accept(self.h, _ptr(a), sz))
+        return a
 
     def rejection_size(self):
         v = C.c_uint64(0)

@@ -18,6 +18,9 @@ ATG_MEM_HOST, ATG_MEM_DEVICE = 0, 1
 ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED, ATG_SPLIT_PARY = 0, 1, 2
 SPLIT_METHODS = {"bisect": ATG_SPLIT_BISECT, "learned": ATG_SPLIT_LEARNED, "pary": ATG_SPLIT_PARY}
 ATG_SAMPLE_BASIC, ATG_SAMPLE_SECTIONED = 0, 1
+ATG_FILL_DIRECT, ATG_FILL_BISECT, ATG_FILL_PARY, ATG_FILL_LEARNED = 0, 1, 2, 3
+FILL_METHODS = {"direct": ATG_FILL_DIRECT, "bisect": ATG_FILL_BISECT, "pary": ATG_FILL_PARY,
+                "learned": ATG_FILL_LEARNED}
 
 DIST = {
     "sine": 1, "best_worst": 2, "uniform": 3, "ramp": 4, "powerlaw1": 5, "powerlaw2": 6,
@@ -72,6 +75,12 @@ SIGNATURES = [
     ("atg_get_lh", _I32, [_P, _P, _P, _P, _P, _P, _P]),
     ("atg_sample", _I32, [_P, _I32, _U64, _U64, _I64, _P, _P, C.POINTER(_U64), _P]),
     ("atg_rejection_build", _I32, [_P, _P]),
+    ("atg_rejection_build_fill", _I32, [_P, _I32, _I32, _P]),
+    ("atg_rejection_fill_a", _I32, [_P, _P]),
+    ("atg_rejection_fill_fallbacks", _I32, [_P, C.POINTER(_U64)]),
+    ("atg_rejection_get_model", _I32, [_P, _P, _P, _I32]),
+    ("atg_rejection_set_model", _I32, [_P, _P, _P, _I32]),
+    ("atg_get_rejection_accept", _I32, [_P, _P, _U64]),
     ("atg_rejection_size", _I32, [_P, C.POINTER(_U64)]),
     ("atg_rejection_weight_sum", _I32, [_P, C.POINTER(_D)]),
     ("atg_get_rejection_array", _I32, [_P, _P, _U64]),

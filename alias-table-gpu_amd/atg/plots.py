@@ -20,6 +20,7 @@ Sweeps (reference function in brackets):
   number_of_samples    type;N;numSamples;gSamplesPerSec [plotNumberOfSamples]
   sampling_speed       type;N;gSamplesPerSec      [plotSamplingSpeed]
   distribution         type;item;sampled          [plotSamplingDistribution]
+  rejection_performance method;N;weight0;time     [plotRejectionPerformance]
 """
 import argparse
 import sys
@@ -237,6 +238,27 @@ def sweep_distribution(ev, out, quick):
         out.write(f"{SAMPLER_NAMES['sectioned']};{i};{int(c)}\n")
 
 
+REJECTION_FILL_NAMES = {"bisect": "binarySearch", "direct": "direct", "pary": "pArySearch", "learned": "learned"}
+
+
+def sweep_rejection_performance(ev, out, quick):
+    """plotRejectionPerformance (PlotConstruction.cuh:18-34): sine weights with
+    item 0 raised to weight0, the mean time of fillA alone after one build
+    (RejectionSampling::benchmarkBuild, RejectionSampling.cu:58-70), per fill."""
+    out.write("method;N;weight0;time\n")
+    for n in ([1_000_000] if quick else [1_000_000, 10_000_000]):
+        base = generate_weights("sine", n)
+        for weight0 in ([1e6, 2e7] if quick else range(1_000_000, 20_000_001, 1_000_000)):
+            w = base.copy()
+            w[0] = weight0
+            t = make_table(n, w=w)
+            for fill, name in REJECTION_FILL_NAMES.items():
+                t.rejection_build(ev.stream, fill=fill)
+                ms = ev.time(lambda: t.rejection_fill(ev.stream), ITERATIONS)
+                out.write(f"{name};{n};{int(weight0)};{ms}\n")
+            t.close()
+
+
 SWEEPS = {
     "build_soa": sweep_build_soa,
     "build_distribution": sweep_build_distribution,
@@ -249,6 +271,7 @@ SWEEPS = {
     "number_of_samples": sweep_number_of_samples,
     "sampling_speed": sweep_sampling_speed,
     "distribution": sweep_distribution,
+    "rejection_performance": sweep_rejection_performance,
 }
 
 

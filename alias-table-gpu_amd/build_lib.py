@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libatg.so")
 UNIT_TESTS = os.path.join(HERE, "cpp", "atg_unit_tests")
-SOURCES = ["atg_build.hip", "atg_sample.hip", "atg_batch.hip", "atg_learned.hip", "atg_noalloc.hip",
+SOURCES = ["atg_build.hip", "atg_sample.hip", "atg_rejfill.hip", "atg_batch.hip", "atg_learned.hip", "atg_noalloc.hip",
            "atg_api.cpp", "atg_generate.cpp", "atg_tune.cpp", "atg_file.cpp"]
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ATG_OFFLOAD_ARCH", "gfx950")

@@ -1,7 +1,8 @@
 // C++ facade over the atg C ABI that mirrors the reference's class surface:
 //   SamplingAlgorithm            construction/SamplingAlgorithm.cuh:30-48
 //   AliasTableSplitGpu<Storage>  construction/aliasTable/buildMethod/AliasTableSplitGpu.cuh:96-144
-//   RejectionSampling            construction/rejectionSampling/RejectionSampling.cuh:13-29
+//   RejectionSampling            construction/rejectionSampling/RejectionSampling.cuh:13-29 (and its
+//                                Direct/BinarySearch/PArySearch subclasses as a fill choice)
 //   Sampler / SamplerGpuBasic / SamplerGpuSectioned / SamplerRejection
 //                                sampling/Sampler.cuh:10-31, sampling/aliasTable/*, sampling/rejectionSampling/*
 // Errors are reported as atg::Error exceptions carrying the atg_status (the
@@ -154,22 +155,39 @@ class AliasTableSplitGpu : public SamplingAlgorithm {
     atg_table* table_ = nullptr;
 };
 
+// The fill of A picks the reference's subclass (RejectionSamplingDirect,
+// RejectionSamplingBinarySearch, RejectionSamplingPArySearch) or the learned
+// inverse-CDF index.
+enum class RejectionFill { Direct = ATG_FILL_DIRECT, BinarySearch = ATG_FILL_BISECT, PArySearch = ATG_FILL_PARY,
+                           Learned = ATG_FILL_LEARNED };
+
 class RejectionSampling : public SamplingAlgorithm {
   public:
-    RejectionSampling(int64_t size, WeightDistribution dist = weightDistributionSine) : SamplingAlgorithm(size, dist) {}
+    RejectionSampling(int64_t size, WeightDistribution dist = weightDistributionSine,
+                      RejectionFill fill = RejectionFill::Direct, int32_t knots = 0)
+        : SamplingAlgorithm(size, dist), fill_(fill), knots_(knots) {}
     ~RejectionSampling() override {
         if (table_) atg_destroy(table_);
     }
-    std::string name() override { return "RejectionSampling"; }
+    std::string name() override {
+        switch (fill_) {
+            case RejectionFill::BinarySearch: return "RejectionSamplingBinarySearch";
+            case RejectionFill::PArySearch: return "RejectionSamplingPArySearch";
+            case RejectionFill::Learned: return "RejectionSamplingLearned";
+            default: return "RejectionSamplingDirect";
+        }
+    }
     void preBuild() override {
         if (!table_) check(atg_create(&table_, N, ATG_F64, ATG_AOS, nullptr));
         check(atg_set_weights(table_, weights.data(), ATG_MEM_HOST, nullptr));
     }
     void build() override {
-        check(atg_rejection_build(table_, nullptr));
+        check(atg_rejection_build_fill(table_, (atg_fill_method)fill_, knots_, nullptr));
         check(atg_sync(table_, nullptr));
         check(atg_rejection_weight_sum(table_, &W));
     }
+    // RejectionSampling::fillA (RejectionSampling.cuh:22): the fill alone.
+    void fillA() { check(atg_rejection_fill_a(table_, nullptr)); }
     // postBuild copies A to the host (RejectionSampling.cu:19-24).
     bool postBuild() override {
         uint64_t n = 0;
@@ -178,10 +196,27 @@ class RejectionSampling : public SamplingAlgorithm {
         check(atg_get_rejection_array(table_, ACpu.data(), n));
         return true;
     }
+    // RejectionSampling::benchmarkBuild (RejectionSampling.cu:58-70): item 0
+    // raised to weightOfOutlier, one build, then the fill alone `iterations`
+    // times; milliseconds per fill.
+    double benchmarkBuild(double weightOfOutlier, int iterations = 10) {
+        weights[0] = weightOfOutlier;
+        preBuild();
+        build();
+        hipDeviceSynchronize();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < iterations; ++i) fillA();
+        check(atg_sync(table_, nullptr));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        postBuild();
+        return ms / iterations;
+    }
     atg_table* handle() { return table_; }
     std::vector<int32_t> ACpu;
 
   private:
+    RejectionFill fill_;
+    int32_t knots_;
     atg_table* table_ = nullptr;
 };
 

@@ -84,6 +84,34 @@ static void testRandomRounds(uint32_t seed) {
     }
 }
 
+// Every fill of A writes the direct fill's A, on every weight distribution
+// and on the outlier weights of RejectionSampling::benchmarkBuild
+// (RejectionSampling.cu:58-70, PlotConstruction.cuh:18-27).
+static void testRejectionFills() {
+    for (int d = 1; d <= MAX_WEIGHT_DISTRIBUTION + 1; ++d) {
+        const bool outlier = d > MAX_WEIGHT_DISTRIBUTION;
+        const int64_t n = outlier ? 100000 : 30011;
+        const WeightDistribution dist = outlier ? weightDistributionSine : (WeightDistribution)d;
+        RejectionSampling direct(n, dist);
+        if (outlier) direct.weights[0] = 2e7;
+        direct.fullBuild();
+        for (RejectionFill f : {RejectionFill::BinarySearch, RejectionFill::PArySearch, RejectionFill::Learned}) {
+            RejectionSampling r(n, dist, f, 256);
+            r.weights = direct.weights;
+            r.fullBuild();
+            const bool ok = r.ACpu == direct.ACpu && r.W == direct.W;
+            std::printf("%s %s n=%lld dist=%d%s |A|=%zu\n", ok ? "ok" : "FAILED:", r.name().c_str(), (long long)n,
+                        (int)dist, outlier ? " (outlier 2e7)" : "", r.ACpu.size());
+            if (!ok) ++failures;
+        }
+    }
+    for (RejectionFill f : {RejectionFill::Direct, RejectionFill::BinarySearch, RejectionFill::Learned}) {
+        RejectionSampling r(1000000, weightDistributionSine, f);
+        const double ms = r.benchmarkBuild(1e7);
+        std::printf("%s: benchmarkBuild(1e7) at N=1e6: %.3f ms per fillA\n", r.name().c_str(), ms);
+    }
+}
+
 // UnitTests::testSamplingDistribution (UnitTests.cu:96-145).
 static void testSamplingDistribution() {
     const uint64_t numSamples = 2000000;
@@ -128,6 +156,7 @@ int main() {
     try {
         testTableCorrectness();
         testRandomRounds(20260101u);
+        testRejectionFills();
         testSamplingDistribution();
     } catch (const Error& e) {
         std::printf("FAILED with status %d: %s\n", (int)e.status, e.what());

@@ -85,6 +85,13 @@ struct atg_table_s {
     uint64_t A_cap = 0;
     uint64_t *d_rej_tot = nullptr, *d_rej_off = nullptr;
     RejTask* d_rej_tasks = nullptr;
+    // search fills: the inclusive counts; the learned fill: its model for
+    // rej_knots segments (buffers for rej_knots_cap)
+    uint32_t* d_rej_incl = nullptr;
+    int32_t* d_rej_knot_items = nullptr;
+    uint32_t* d_rej_knot_err = nullptr;
+    int32_t rej_knots_cap = 0, rej_knots = 0;
+    int32_t rej_fill = ATG_FILL_DIRECT;
     // sectioned rejection sampling: |A| read back once per rejection build;
     // section masses, their prefix and the section counts for rej_sec_cap
     // sections, the masses valid for rej_sec_SS slots per section
@@ -174,7 +181,8 @@ static void free_table(atg_table* t) {
                     t->d_stage_i, t->d_Li, t->d_Hi,
                     t->d_Lw, t->d_Lp, t->d_Hw, t->d_Hp, t->d_rows, t->d_tw, t->d_ta, t->d_splits, t->d_knots,
                     t->d_checksum, t->d_rej_state, t->d_A, t->d_rej_tot, t->d_rej_off, t->d_rej_tasks,
-                    t->d_sec_counts, t->d_rej_mass, t->d_rej_pre, t->d_rej_sec_counts};
+                    t->d_sec_counts, t->d_rej_mass, t->d_rej_pre, t->d_rej_sec_counts, t->d_rej_incl,
+                    t->d_rej_knot_items, t->d_rej_knot_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (t->events)
@@ -678,9 +686,46 @@ atg_status atg_sample(atg_table* t, atg_sample_kind kind, uint64_t count, uint64
     return ATG_OK;
 }
 
+// The n the search fills accept: |A| <= A_cap = 2n + 4096 stays below 2^32,
+// so the inclusive counts fit 32 bits.
+static constexpr int64_t kMaxSearchFillN = 0x7FFFF000LL;
+static constexpr int32_t kDefaultFillKnots = 4096, kMaxFillKnots = 1 << 22;
+
+static RejArgs rej_args(atg_table* t) {
+    RejArgs r{};
+    r.weights = t->d_weights;
+    r.dtype_f32 = t->dtype == ATG_F32;
+    r.n = t->n;
+    r.st = t->d_rej_state;
+    r.nb = rejection_blocks(t->n);
+    r.block_tot = t->d_rej_tot;
+    r.block_off = t->d_rej_off;
+    r.incl = t->d_rej_incl;
+    r.knot_items = t->d_rej_knot_items;
+    r.knot_err = t->d_rej_knot_err;
+    r.K = t->rej_knots;
+    r.A = t->d_A;
+    r.A_cap = t->A_cap;
+    r.tasks = t->d_rej_tasks;
+    r.tasks_cap = rejection_task_cap(t->A_cap);
+    return r;
+}
+
 atg_status atg_rejection_build(atg_table* t, void* stream) {
+    return atg_rejection_build_fill(t, ATG_FILL_DIRECT, 0, stream);
+}
+
+atg_status atg_rejection_build_fill(atg_table* t, atg_fill_method fill, int32_t knots, void* stream) {
     if (!t) return fail(ATG_EINVAL, "atg_rejection_build: NULL table");
     if (!t->weights_set) return fail(ATG_ESTATE, "atg_rejection_build: weights not set");
+    if (fill < ATG_FILL_DIRECT || fill > ATG_FILL_LEARNED)
+        return fail(ATG_EINVAL, "atg_rejection_build_fill: unknown fill %d", (int)fill);
+    if (knots == 0) knots = kDefaultFillKnots;
+    if (fill == ATG_FILL_LEARNED && (knots < 1 || knots > kMaxFillKnots))
+        return fail(ATG_EINVAL, "atg_rejection_build_fill: %d knots outside [1, %d]", knots, kMaxFillKnots);
+    if (fill != ATG_FILL_DIRECT && t->n > kMaxSearchFillN)
+        return fail(ATG_EINVAL, "atg_rejection_build_fill: n=%lld exceeds %lld for the search fills",
+                    (long long)t->n, (long long)kMaxSearchFillN);
     hipStream_t s = (hipStream_t)stream;
     const int64_t nb = rejection_blocks(t->n);
     if (!t->d_A) {
@@ -707,28 +752,94 @@ atg_status atg_rejection_build(atg_table* t, void* stream) {
         t->d_rej_off = off;
         t->d_rej_tasks = tasks;
     }
+    if (fill != ATG_FILL_DIRECT && !t->d_rej_incl && dalloc(&t->d_rej_incl, (size_t)t->n) != hipSuccess) {
+        (void)hipGetLastError();
+        t->d_rej_incl = nullptr;
+        return fail(ATG_ENOMEM, "atg_rejection_build_fill: inclusive counts");
+    }
+    if (fill == ATG_FILL_LEARNED && knots > t->rej_knots_cap) {
+        // Both model arrays or neither.
+        if (t->d_rej_knot_items) (void)hipFree(t->d_rej_knot_items);
+        if (t->d_rej_knot_err) (void)hipFree(t->d_rej_knot_err);
+        t->d_rej_knot_items = nullptr;
+        t->d_rej_knot_err = nullptr;
+        t->rej_knots_cap = 0;
+        if (dalloc(&t->d_rej_knot_items, (size_t)knots + 1) != hipSuccess ||
+            dalloc(&t->d_rej_knot_err, (size_t)knots) != hipSuccess) {
+            (void)hipGetLastError();
+            if (t->d_rej_knot_items) (void)hipFree(t->d_rej_knot_items);
+            t->d_rej_knot_items = nullptr;
+            t->d_rej_knot_err = nullptr;
+            return fail(ATG_ENOMEM, "atg_rejection_build_fill: model of %d knots", knots);
+        }
+        t->rej_knots_cap = knots;
+    }
     t->last_stream = s;
+    t->rej_built = false;
+    t->rej_fill = fill;
+    t->rej_knots = fill == ATG_FILL_LEARNED ? knots : 0;
     // W: the canonical sum (RejectionSampling.cu:36 uses the host sum), into
     // the rejection structure's own state.
     BuildArgs ba = t->args();
     ba.st = t->d_rej_state;
     HIPCHK(launch_sum(ba, s));
-    RejArgs r{};
-    r.weights = t->d_weights;
-    r.dtype_f32 = t->dtype == ATG_F32;
-    r.n = t->n;
-    r.st = t->d_rej_state;
-    r.nb = nb;
-    r.block_tot = t->d_rej_tot;
-    r.block_off = t->d_rej_off;
-    r.A = t->d_A;
-    r.A_cap = t->A_cap;
-    r.tasks = t->d_rej_tasks;
-    r.tasks_cap = rejection_task_cap(t->A_cap);
-    HIPCHK(launch_rejection_build(r, s));
+    HIPCHK(launch_rejection_build(rej_args(t), fill, s));
     t->rej_built = true;
     t->rej_size_known = false;
     t->rej_sec_valid = false;
+    return ATG_OK;
+}
+
+atg_status atg_rejection_fill_a(atg_table* t, void* stream) {
+    if (!t) return fail(ATG_EINVAL, "atg_rejection_fill_a: NULL table");
+    if (!t->rej_built) return fail(ATG_ESTATE, "atg_rejection_fill_a: rejection structure not built");
+    hipStream_t s = (hipStream_t)stream;
+    t->last_stream = s;
+    HIPCHK(launch_rejection_fill(rej_args(t), t->rej_fill, s));
+    return ATG_OK;
+}
+
+atg_status atg_rejection_fill_fallbacks(atg_table* t, uint64_t* fallbacks) {
+    if (!t || !fallbacks) return fail(ATG_EINVAL, "atg_rejection_fill_fallbacks: NULL argument");
+    if (!t->rej_built) return fail(ATG_ESTATE, "atg_rejection_fill_fallbacks: rejection structure not built");
+    DevState st;
+    HIPCHK(hipStreamSynchronize(t->last_stream));
+    HIPCHK(hipMemcpy(&st, t->d_rej_state, sizeof(st), hipMemcpyDeviceToHost));
+    *fallbacks = st.learned_fallbacks;
+    return state_status(st);
+}
+
+static atg_status check_model_args(atg_table* t, const void* q, const void* e, int32_t knots, const char* who) {
+    if (!t || !q || !e) return fail(ATG_EINVAL, "%s: NULL argument", who);
+    if (!t->rej_built || t->rej_fill != ATG_FILL_LEARNED)
+        return fail(ATG_ESTATE, "%s: no learned fill built", who);
+    if (knots != t->rej_knots) return fail(ATG_EINVAL, "%s: %d knots, the model has %d", who, knots, t->rej_knots);
+    return ATG_OK;
+}
+
+atg_status atg_rejection_get_model(atg_table* t, int32_t* knot_items, uint32_t* err_bounds, int32_t knots) {
+    atg_status rc = check_model_args(t, knot_items, err_bounds, knots, "atg_rejection_get_model");
+    if (rc != ATG_OK) return rc;
+    HIPCHK(hipStreamSynchronize(t->last_stream));
+    HIPCHK(hipMemcpy(knot_items, t->d_rej_knot_items, sizeof(int32_t) * ((size_t)knots + 1), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(err_bounds, t->d_rej_knot_err, sizeof(uint32_t) * (size_t)knots, hipMemcpyDeviceToHost));
+    return ATG_OK;
+}
+
+atg_status atg_rejection_set_model(atg_table* t, const int32_t* knot_items, const uint32_t* err_bounds,
+                                   int32_t knots) {
+    atg_status rc = check_model_args(t, knot_items, err_bounds, knots, "atg_rejection_set_model");
+    if (rc != ATG_OK) return rc;
+    // The fill reads incl only between knot items: they must be positions,
+    // in order.
+    for (int32_t k = 0; k <= knots; ++k) {
+        if (knot_items[k] < 0 || knot_items[k] >= t->n || (k > 0 && knot_items[k] < knot_items[k - 1]))
+            return fail(ATG_EINVAL, "atg_rejection_set_model: knot item %d = %d out of order or range", k,
+                        knot_items[k]);
+    }
+    HIPCHK(hipStreamSynchronize(t->last_stream));
+    HIPCHK(hipMemcpy(t->d_rej_knot_items, knot_items, sizeof(int32_t) * ((size_t)knots + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->d_rej_knot_err, err_bounds, sizeof(uint32_t) * (size_t)knots, hipMemcpyHostToDevice));
     return ATG_OK;
 }
 
@@ -832,6 +943,18 @@ atg_status atg_get_rejection_array(atg_table* t, int32_t* A, uint64_t cap) {
     std::vector<RejEntry> e((size_t)sz);
     if (sz) HIPCHK(hipMemcpy(e.data(), t->d_A, sizeof(RejEntry) * sz, hipMemcpyDeviceToHost));
     for (uint64_t i = 0; i < sz; ++i) A[i] = e[i].item;
+    return ATG_OK;
+}
+
+atg_status atg_get_rejection_accept(atg_table* t, double* accept, uint64_t cap) {
+    if (!accept) return fail(ATG_EINVAL, "atg_get_rejection_accept: NULL argument");
+    uint64_t sz = 0;
+    atg_status rc = atg_rejection_size(t, &sz);
+    if (rc != ATG_OK) return rc;
+    if (cap < sz) return fail(ATG_EINVAL, "atg_get_rejection_accept: capacity %llu < %llu", (unsigned long long)cap, (unsigned long long)sz);
+    std::vector<RejEntry> e((size_t)sz);
+    if (sz) HIPCHK(hipMemcpy(e.data(), t->d_A, sizeof(RejEntry) * sz, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < sz; ++i) accept[i] = e[i].accept;
     return ATG_OK;
 }
 

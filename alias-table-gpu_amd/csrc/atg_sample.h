@@ -39,6 +39,16 @@ struct SampleArgs {
     uint64_t* sec_counts;  // sectioned: 2 nsec values, count and first output of each section
 };
 
+// The rejection structure is scanned in blocks of REJ_ITEMS items of
+// REJ_THREADS threads (block totals, their scan, then the per-block fills).
+#define REJ_ITEMS 4096
+#define REJ_THREADS 256
+
+// Fills of A (RejectionSampling::fillA): the item runs written directly, a
+// bisection per slot, a wave-wide p-ary narrowing then a bisection per slot,
+// and the learned inverse-CDF index.
+enum RejFill : int { REJ_FILL_DIRECT = 0, REJ_FILL_BISECT = 1, REJ_FILL_PARY = 2, REJ_FILL_LEARNED = 3 };
+
 struct RejArgs {
     const void* weights;
     int dtype_f32;
@@ -47,7 +57,10 @@ struct RejArgs {
     int64_t nb;            // blocks of 4096 items
     uint64_t* block_tot;   // nb
     uint64_t* block_off;   // nb
-    uint64_t* incl_out;    // n inclusive offsets (nullable)
+    uint32_t* incl;        // search fills: n inclusive counts (|A| < 2^32 for the n they accept)
+    int32_t* knot_items;   // learned fill: K + 1 knot items of the inverse-CDF model
+    uint32_t* knot_err;    // learned fill: K per-segment error bounds
+    int32_t K;             // learned fill: segments
     RejEntry* A;          // |A| <= A_cap slots
     uint64_t A_cap;
     unsigned long long* trials;  // nullable
@@ -72,7 +85,14 @@ hipError_t launch_rejection_sample_sectioned(const RejSecArgs& a, const SampleAr
 hipError_t set_rej_sample_lds_limit(int bytes);
 hipError_t launch_sample_sectioned(const SampleArgs& a, hipStream_t s);
 hipError_t set_sample_lds_limit(int bytes);
-hipError_t launch_rejection_build(const RejArgs& a, hipStream_t s);
+// The counts, their scan and the fill (with the incl pass and the model fit
+// the search fills need); launch_rejection_fill repeats the fill alone.
+hipError_t launch_rejection_build(const RejArgs& a, int fill, hipStream_t s);
+hipError_t launch_rejection_fill(const RejArgs& a, int fill, hipStream_t s);
+// The inclusive counts (and with model, the learned index fitted on them),
+// then a search fill (atg_rejfill.hip).
+hipError_t launch_rejection_incl(const RejArgs& a, bool model, hipStream_t s);
+hipError_t launch_rejection_search_fill(const RejArgs& a, int fill, hipStream_t s);
 hipError_t launch_rejection_sample(const RejArgs& a, const SampleArgs& sa, hipStream_t s);
 int64_t rejection_blocks(int64_t n);
 uint32_t rejection_task_cap(uint64_t A_cap);

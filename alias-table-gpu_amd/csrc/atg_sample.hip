@@ -177,8 +177,6 @@ __global__ __launch_bounds__(SECTIONED_THREADS) void sample_sectioned(SampleArgs
 // is scanned with integers (exact in any order): per-block totals, one block
 // scanning the totals, then each block writes its items' slots of A; the
 // slots of long runs are cut into tasks that the whole grid writes.
-#define REJ_ITEMS 4096
-#define REJ_THREADS 256
 #define REJ_WAVE 64     // runs up to this are written by their wave, longer ones as tasks
 #define REJ_TASK 4096   // slots per task
 #define REJ_LONG_BLOCKS 1024
@@ -261,6 +259,7 @@ __global__ __launch_bounds__(1024) void rej_scan_totals(RejArgs a) {
         // (or is empty, past nb, after every non-empty run).
         a.st->rej_size = off;
         a.st->rej_tasks = 0;
+        a.st->learned_fallbacks = 0;  // waves of the learned fill that fell back to bisection
         if (off > a.A_cap) atomicOr(&a.st->error, ERR_BAD_SUM);
     }
 }
@@ -315,7 +314,6 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
         const int32_t item = (int32_t)(base + k);
         const double x = ((double)a.n * (double)wr[u]) / W;
         const double frac = x - floor(x);
-        if (k < cnt && a.incl_out) a.incl_out[base + k] = first + cu;
         // The round's slots in the wave, long runs counted by their first slot
         // only: lane v of each pass writes the v-th of them, owned by the
         // last item whose local start is <= v (a search over the 64 starts).
@@ -622,20 +620,35 @@ hipError_t set_sample_lds_limit(int bytes) {
                                bytes);
 }
 
-hipError_t launch_rejection_build(const RejArgs& a, hipStream_t s) {
+// rej_fill adds the long runs' tasks to a counter that the scan of the block
+// totals clears; a repeated fill clears it itself.
+__global__ void rej_clear_tasks(DevState* st) { st->rej_tasks = 0; }
+
+static hipError_t launch_direct_fill(const RejArgs& a, hipStream_t s) {
     const unsigned nb = (unsigned)a.nb;
-    if (a.dtype_f32) {
-        hipLaunchKernelGGL(rej_block_totals<float>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
-        hipLaunchKernelGGL(rej_scan_totals, dim3(1), dim3(1024), 0, s, a);
-        hipLaunchKernelGGL(rej_fill<float>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
-        hipLaunchKernelGGL(rej_fill_long, dim3(REJ_LONG_BLOCKS), dim3(256), 0, s, a);
-    } else {
-        hipLaunchKernelGGL(rej_block_totals<double>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
-        hipLaunchKernelGGL(rej_scan_totals, dim3(1), dim3(1024), 0, s, a);
-        hipLaunchKernelGGL(rej_fill<double>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
-        hipLaunchKernelGGL(rej_fill_long, dim3(REJ_LONG_BLOCKS), dim3(256), 0, s, a);
-    }
+    if (a.dtype_f32) hipLaunchKernelGGL(rej_fill<float>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
+    else hipLaunchKernelGGL(rej_fill<double>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
+    hipLaunchKernelGGL(rej_fill_long, dim3(REJ_LONG_BLOCKS), dim3(256), 0, s, a);
     return hipGetLastError();
+}
+
+hipError_t launch_rejection_build(const RejArgs& a, int fill, hipStream_t s) {
+    const unsigned nb = (unsigned)a.nb;
+    if (a.dtype_f32) hipLaunchKernelGGL(rej_block_totals<float>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
+    else hipLaunchKernelGGL(rej_block_totals<double>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
+    hipLaunchKernelGGL(rej_scan_totals, dim3(1), dim3(1024), 0, s, a);
+    if (fill == REJ_FILL_DIRECT) return launch_direct_fill(a, s);
+    hipError_t e = launch_rejection_incl(a, fill == REJ_FILL_LEARNED, s);
+    if (e != hipSuccess) return e;
+    return launch_rejection_search_fill(a, fill, s);
+}
+
+hipError_t launch_rejection_fill(const RejArgs& a, int fill, hipStream_t s) {
+    if (fill == REJ_FILL_DIRECT) {
+        hipLaunchKernelGGL(rej_clear_tasks, dim3(1), dim3(1), 0, s, a.st);
+        return launch_direct_fill(a, s);
+    }
+    return launch_rejection_search_fill(a, fill, s);
 }
 
 int64_t rejection_blocks(int64_t n) { return (n + REJ_ITEMS - 1) / REJ_ITEMS; }

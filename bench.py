@@ -226,6 +226,44 @@ def bench_learned(args, stream):
     res["bit_identical"] = all(bool(np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64)) and
                                     np.array_equal(a[1], b[1])) for b in (tabs["learned"], tabs["pary"]))
     log(f"[bench] learned vs plain: {res}")
+    if not args.no_rejection_fills:
+        res["rejection_fills"] = bench_rejection_fills(args, stream, w, f"N={n:.0e} heavy_mix (1% heavy, 90% mass)")
+    return res
+
+
+def bench_rejection_fills(args, stream, w, label):
+    """The fills of the rejection array on one weight set (RejectionSampling::
+    fillA by subclass, plus the learned inverse-CDF index): the whole
+    rejection build and fillA alone per fill, the learned model's error
+    bounds and fallbacks."""
+    import numpy as np
+    import torch
+    import atg
+    n, K = len(w), args.learned_segments
+    t = atg.Table(n)
+    t.set_weights(w, stream)
+    res = {"workload": label, "knots": K}
+    reps = max(3, args.steps // 4)
+    for fill in ("direct", "bisect", "pary", "learned"):
+        t.rejection_build(stream, fill=fill, knots=K)
+        torch.cuda.synchronize()
+        bms, _ = time_events(stream, lambda: t.rejection_build(stream, fill=fill, knots=K), reps)
+        fms, _ = time_events(stream, lambda: t.rejection_fill(stream), reps)
+        A = t.rejection_size()
+        r = {"build_ms": round(bms, 4), "fill_ms": round(fms, 4), "fill_gslots_s": round(A / (fms * 1e-3) / 1e9, 3),
+             # A written once (16 B a slot) over the fill's time
+             "fill_A_write_frac": round(16 * A / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if fill == "learned":
+            r["fallbacks"] = t.rejection_fill_fallbacks()
+            _, e = t.rejection_model(K)
+            r["err_bound_median"], r["err_bound_max"] = int(np.median(e)), int(e.max())
+        res[fill] = r
+        log(f"[bench] rejection fill {fill}: build {bms:.3f} ms, fillA {fms:.3f} ms")
+    res["A_size"] = A
+    res["learned_vs_bisect_fill"] = round(res["bisect"]["fill_ms"] / res["learned"]["fill_ms"], 3)
+    res["learned_vs_bisect_build"] = round(res["bisect"]["build_ms"] / res["learned"]["build_ms"], 3)
+    res["pary_vs_bisect_fill"] = round(res["bisect"]["fill_ms"] / res["pary"]["fill_ms"], 3)
+    t.close()
     return res
 
 
@@ -375,6 +413,8 @@ def main():
     ap.add_argument("--learned-chains", type=int, default=12)
     ap.add_argument("--sample-section", type=int, default=4096)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-rejection-fills", action="store_true",
+                    help="skip the fills of the rejection array (direct, bisection, p-ary, learned)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-batch-learned", action="store_true", help="skip the 16,384-table FP32 batch")
     ap.add_argument("--batch-section", type=int, default=2048, help="section size of every batched table")
@@ -465,6 +505,9 @@ def main():
     if not args.no_sampling:
         result["sampling"] = bench_sampling(args, stream, t, n)
     del t
+    torch.cuda.empty_cache()
+    if not args.no_rejection_fills:
+        result["rejection_fills"] = bench_rejection_fills(args, stream, w, f"N={n:.0e} Pareto(1.1)")
     torch.cuda.empty_cache()
     if not args.no_reference_shape:
         result["reference_shape"] = bench_reference_shape(args, stream, w)

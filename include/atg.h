@@ -232,6 +232,46 @@ atg_status atg_sample(atg_table* t, atg_sample_kind kind, uint64_t count, uint64
  * A (item i repeated floor(N w_i / W) + 1 times) from the table's weights and
  * canonical W.  Requires atg_set_weights (a table build is not needed). */
 atg_status atg_rejection_build(atg_table* t, void* stream);
+
+/* How A is filled once the counts are scanned (RejectionSampling::fillA,
+ * RejectionSampling.cuh:22; the reference picks it by subclass).  Every fill
+ * writes the same A.
+ *   DIRECT   each item's run written by its wave, long runs cut into tasks
+ *            (RejectionSamplingDirect.cu:12-28); the default
+ *   BISECT   a bisection over the inclusive counts per slot
+ *            (RejectionSamplingBinarySearch.cu:12-33)
+ *   PARY     a wave-wide p-ary narrowing, then a bisection per slot
+ *            (RejectionSamplingPArySearch.cu:12-123)
+ *   LEARNED  a piecewise-linear inverse-CDF model (slot -> item) fitted on the
+ *            inclusive counts, with exact per-segment error bounds; each
+ *            prediction is corrected by a verified search inside its bound,
+ *            a miss falls back to the bisection (no reference code) */
+typedef enum { ATG_FILL_DIRECT = 0, ATG_FILL_BISECT = 1, ATG_FILL_PARY = 2, ATG_FILL_LEARNED = 3 } atg_fill_method;
+
+/* RejectionSampling::build (RejectionSampling.cu:35-56) with the chosen fill;
+ * knots = the learned model's segments (0 = 4096, at most 2^22; ignored by
+ * the other fills).  The search fills keep the n inclusive counts (4 B each)
+ * and accept n <= 0x7FFFF000 (|A| < 2^32). */
+atg_status atg_rejection_build_fill(atg_table* t, atg_fill_method fill, int32_t knots, void* stream);
+/* RejectionSampling::fillA alone (the loop of benchmarkBuild,
+ * RejectionSampling.cu:58-70): repeats the last build's fill over its counts
+ * (and model); A comes out the same. */
+atg_status atg_rejection_fill_a(atg_table* t, void* stream);
+/* Waves of the learned fill whose prediction missed and fell back to the
+ * bisection, summed over the fills since the last build. */
+atg_status atg_rejection_fill_fallbacks(atg_table* t, uint64_t* fallbacks);
+/* The learned model of the last build (its fill LEARNED with `knots`
+ * segments): knot_items[knots + 1] and err_bounds[knots], host arrays. */
+atg_status atg_rejection_get_model(atg_table* t, int32_t* knot_items, uint32_t* err_bounds, int32_t knots);
+/* Replaces that model (same knots; knot items non-decreasing in [0, n-1],
+ * any bounds) for the next atg_rejection_fill_a: a model with wrong bounds
+ * changes the fallback count, never A.  For tests of the correction. */
+atg_status atg_rejection_set_model(atg_table* t, const int32_t* knot_items, const uint32_t* err_bounds,
+                                   int32_t knots);
+/* The acceptance value of every slot of A (frac(N w / W) at the first slot
+ * of a run, 1.0 elsewhere), host array of capacity >= |A|. */
+atg_status atg_get_rejection_accept(atg_table* t, double* accept, uint64_t capacity);
+
 atg_status atg_rejection_size(atg_table* t, uint64_t* a_size);
 /* RejectionSampling::W (SamplingAlgorithm.cuh:44): the canonical weight sum
  * the rejection structure was built with (kept apart from the alias build's). */

@@ -1058,3 +1058,51 @@ extern "C" void oracle_split_inverse(int64_t p, int64_t nToSplit, double wToSpli
         }
     }
 }
+
+/* fillABinarySearch (RejectionSamplingBinarySearch.cu:12-33): slot s holds the
+ * item found by bisection over the inclusive counts, the first i with
+ * incl[i] > s.  A second restatement of the fill beside the run-writing one
+ * of oracle_rejection_build. */
+extern "C" void oracle_rejection_fill_bisect(const uint64_t* incl, int64_t n, uint64_t A_size, int32_t* A) {
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < (int64_t)A_size; ++s) {
+        int64_t left = 0, right = n;
+        while (right - left > 1) {
+            const int64_t c = (left + right) / 2;
+            if (incl[c] > (uint64_t)s) right = c;
+            else left = c;
+        }
+        A[s] = (int32_t)(incl[left] <= (uint64_t)s ? right : left);
+    }
+}
+
+/* The learned inverse-CDF model of A over the inclusive counts (no reference
+ * code; it predicts the item of a slot, the quantity fillABinarySearch
+ * searches for).  With M = |A| - 1 and K segments:
+ *   segment of slot s      seg(s) = 0 if M = 0, else min(K - 1, floor(s K / M))
+ *   knot slot k (0..K)     s_k = ceil(k M / K)   (so seg(s) = k iff s_k <= s < s_{k+1})
+ *   knot item              q_k = item of slot s_k
+ *   prediction in seg k    pred(s) = q_k + floor((s - s_k)(q_{k+1} - q_k) / (s_{k+1} - s_k))
+ *                          (q_k when s_{k+1} = s_k)
+ *   error bound            e_k = max over the slots s of segment k of |item(s) - pred(s)|
+ * all in exact integer arithmetic.  Restated here by brute force over every
+ * slot; the library computes e_k from the two ends of each item's run. */
+extern "C" void oracle_rejection_model(const uint64_t* incl, int64_t n, uint64_t A_size, int32_t K,
+                                       int32_t* q, uint32_t* err) {
+    const uint64_t M = A_size - 1;
+    auto kslot = [&](uint64_t k) { return (k * M + (uint64_t)K - 1) / (uint64_t)K; };
+    for (int32_t k = 0; k <= K; ++k) {
+        const uint64_t s = kslot((uint64_t)k);
+        q[k] = (int32_t)(std::upper_bound(incl, incl + n, s) - incl);
+    }
+    std::fill(err, err + K, 0u);
+    int64_t item = 0;
+    for (uint64_t s = 0; s < A_size; ++s) {
+        while (incl[item] <= s) ++item;
+        const uint64_t k = M == 0 ? 0 : std::min<uint64_t>((uint64_t)K - 1, s * (uint64_t)K / M);
+        const uint64_t sk = kslot(k), sk1 = kslot(k + 1);
+        const int64_t pred = sk1 > sk ? q[k] + (int64_t)((s - sk) * (uint64_t)(q[k + 1] - q[k]) / (sk1 - sk)) : q[k];
+        const uint32_t e = (uint32_t)(item > pred ? item - pred : pred - item);
+        err[k] = std::max(err[k], e);
+    }
+}

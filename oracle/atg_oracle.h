@@ -163,6 +163,10 @@ void oracle_sample_rejection_sectioned(const int32_t* A, uint64_t A_size, const 
                                        int64_t section_slots, uint64_t total, uint64_t seed,
                                        const int64_t* sections /* nullable: all */, int64_t nsel,
                                        int32_t* out /* total */, uint64_t* first_out, uint64_t* cnt_out);
+/* Search fills of A and the learned inverse-CDF model (see atg_oracle.cpp). */
+void oracle_rejection_fill_bisect(const uint64_t* incl, int64_t n, uint64_t A_size, int32_t* A);
+void oracle_rejection_model(const uint64_t* incl, int64_t n, uint64_t A_size, int32_t K,
+                            int32_t* knot_items /* K + 1 */, uint32_t* err_bounds /* K */);
 
 #ifdef __cplusplus
 }

@@ -351,3 +351,25 @@ def split_inverse(p, nToSplit, wToSplit, Lp, Hp, Hw):
     L.oracle_split_inverse(int(p), int(nToSplit), float(wToSplit), ptr(Lp), len(Lp), ptr(Hp), ptr(Hw), len(Hp),
                            ptr(si), ptr(sj), ptr(ss))
     return si, sj, ss
+
+
+L.oracle_rejection_fill_bisect.argtypes = [P, C.c_int64, C.c_uint64, P]
+L.oracle_rejection_model.argtypes = [P, C.c_int64, C.c_uint64, C.c_int32, P, P]
+
+
+def rejection_fill_bisect(incl):
+    """A by the reference's per-slot bisection over the inclusive counts."""
+    incl = np.ascontiguousarray(incl, dtype=np.uint64)
+    A = np.empty(int(incl[-1]), np.int32)
+    L.oracle_rejection_fill_bisect(ptr(incl), len(incl), len(A), ptr(A))
+    return A
+
+
+def rejection_model(incl, knots):
+    """Learned inverse-CDF model over the inclusive counts: knot items (knots
+    + 1) and per-segment error bounds (knots), by brute force over every slot."""
+    incl = np.ascontiguousarray(incl, dtype=np.uint64)
+    q = np.empty(knots + 1, np.int32)
+    e = np.empty(knots, np.uint32)
+    L.oracle_rejection_model(ptr(incl), len(incl), int(incl[-1]), int(knots), ptr(q), ptr(e))
+    return q, e

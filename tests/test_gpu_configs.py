@@ -12,6 +12,9 @@
   with the oracle on a few hundred sections (the first and the last among
   them); the rejection array A bit for bit and 1e7 rejection samples; 1e7
   basic samples.
+* N = 1e8 with 1% of the items holding 90% of the mass: the rejection array
+  through the learned inverse-CDF fill (4096 segments), the p-ary and the
+  bisection fills, each equal to the oracle's; the model equal to the oracle's.
 """
 import numpy as np
 import pytest
@@ -160,3 +163,39 @@ def test_rejection_and_basic_headline(gpu, oracle, headline):
     for s, f, c in zip(sel.tolist(), first.tolist(), cnt.tolist()):
         assert np.array_equal(out[f:f + c].cpu().numpy(), exp[f:f + c]), f"section {s}"
     log(f"sectioned rejection: {sel.size} sections, {int(cnt.sum())} samples compared")
+
+
+def test_learned_inverse_cdf_fill_heavy_mix_1e8(gpu, oracle):
+    """N = 1e8 with 1% of the items holding 90% of the mass, the rejection
+    array filled through the 4096-segment inverse-CDF model fitted on the
+    build's own inclusive counts, through the p-ary and the plain bisection
+    fills: A (items and acceptance bits) equals the oracle's for each, the
+    model equals the oracle's brute-force fit, and no prediction falls back."""
+    n, K = N, 4096
+    w = atg.generate_weights("heavy_mix", n, seed=42)
+    t = atg.Table(n)
+    t.set_weights(w)
+    ref = None
+    for fill in ("learned", "pary", "bisect"):
+        t.rejection_build(fill=fill, knots=K)
+        W = t.rejection_weight_sum()
+        if ref is None:
+            incl, A_ref = oracle.rejection_build(w, W)
+            x = (float(n) * w) / W
+            first = np.concatenate([[0], incl[:-1]]).astype(np.int64)
+            acc_ref = np.ones(A_ref.size)
+            acc_ref[first] = x - np.floor(x)
+            del x, first
+            ref = W
+            q, e = t.rejection_model(K)
+            q_ref, e_ref = oracle.rejection_model(incl, K)
+            assert np.array_equal(q, q_ref) and np.array_equal(e, e_ref)
+            assert t.rejection_fill_fallbacks() == 0
+            log(f"|A| = {A_ref.size}, model bounds: median {int(np.median(e))}, max {int(e.max())}")
+        assert W == ref
+        A = t.rejection_array()
+        assert A.size == A_ref.size and np.array_equal(A, A_ref), fill
+        del A
+        assert np.array_equal(bits(t.rejection_accept()), bits(acc_ref)), fill
+        log(f"{fill} fill: A equal to the oracle's")
+    t.close()

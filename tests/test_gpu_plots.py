@@ -20,6 +20,7 @@ HEADERS = {
     "number_of_samples": "type;N;numSamples;gSamplesPerSec",
     "sampling_speed": "type;N;gSamplesPerSec",
     "distribution": "type;item;sampled",
+    "rejection_performance": "method;N;weight0;time",
 }
 
 
@@ -35,7 +36,7 @@ def test_sweep_writes_its_schema(gpu, sweep):
         f = row.split(";")
         assert len(f) == len(cols), row
         for name, v in zip(cols, f):
-            if name in ("duration", "gSamplesPerSec"):
+            if name in ("duration", "gSamplesPerSec", "time"):
                 assert float(v) > 0, row
     if sweep == "distribution":
         assert sum(int(r.split(";")[2]) for r in lines[1:]) == 100_000

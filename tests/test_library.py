@@ -67,7 +67,7 @@ def test_generate_rejects_bad_distribution():
 def test_plots_harness_lists_the_reference_sweeps():
     from atg import plots
     assert {"build_soa", "thread_number", "split_variants", "with_transfer", "light_vs_heavy",
-            "sampler_method", "sampling_speed", "distribution"} <= set(plots.SWEEPS)
+            "sampler_method", "sampling_speed", "distribution", "rejection_performance"} <= set(plots.SWEEPS)
     import subprocess, sys, os
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     r = subprocess.run([sys.executable, "-m", "atg.plots", "--help"], capture_output=True, text=True,

@@ -418,6 +418,67 @@ def test_rejection_array_brute_force(oracle):
     assert np.array_equal(A, np.repeat(np.arange(len(w)), counts))
 
 
+def _outlier_sine(n, weight0):
+    # The reference's rejection construction plot: sine weights with item 0
+    # raised to weightOfOutlier (RejectionSampling.cu:58-60, PlotConstruction.cuh:18-27).
+    w = atg.generate_weights("sine", n)
+    w[0] = weight0
+    return w
+
+
+@pytest.mark.parametrize("case", ["pareto", "outlier", "uniform", "one_heavy", "tiny"])
+def test_rejection_fill_bisect_equals_runs(oracle, case):
+    # The reference's per-slot bisection (RejectionSamplingBinarySearch.cu:12-33)
+    # and the run-writing fill give one A, also with a run of most of A.
+    w = {"pareto": lambda: atg.generate_weights("pareto", 30000, param=1.1),
+         "outlier": lambda: _outlier_sine(10000, 2e7),
+         "uniform": lambda: np.full(4096, 0.5),
+         "one_heavy": lambda: atg.generate_weights("one_heavy", 5000),
+         "tiny": lambda: np.array([3.0])}[case]()
+    W = float(np.sum(w))
+    incl, A = oracle.rejection_build(w, W)
+    assert np.array_equal(oracle.rejection_fill_bisect(incl), A)
+
+
+def _model_pred(q, A_size, K, s):
+    """pred(s) of the inverse-CDF model from its knot items (definition in
+    oracle_rejection_model), vectorised, and the segment of each slot."""
+    M = A_size - 1
+    s = s.astype(np.uint64)
+    k = np.zeros_like(s) if M == 0 else np.minimum(K - 1, s * np.uint64(K) // np.uint64(M))
+    kslot = lambda kk: (kk * np.uint64(M) + np.uint64(K - 1)) // np.uint64(K)
+    sk, sk1 = kslot(k), kslot(k + np.uint64(1))
+    qk, qk1 = q[k.astype(np.int64)].astype(np.uint64), q[(k + np.uint64(1)).astype(np.int64)].astype(np.uint64)
+    span = np.maximum(sk1 - sk, np.uint64(1))
+    pred = np.where(sk1 > sk, qk + (s - sk) * (qk1 - qk) // span, qk)
+    return k.astype(np.int64), pred.astype(np.int64)
+
+
+@pytest.mark.parametrize("case,K", [("pareto", 64), ("pareto", 4096), ("outlier", 37), ("heavy_mix", 512),
+                                    ("tiny", 8), ("uniform", 1000)])
+def test_rejection_model_knots_and_error_bounds(oracle, case, K):
+    # The knots are the items of the knot slots (a searchsorted over the
+    # inclusive counts); every slot's item lies within its segment's bound
+    # of the prediction, and every non-empty segment attains its bound.
+    w = {"pareto": lambda: atg.generate_weights("pareto", 40000, param=1.1),
+         "outlier": lambda: _outlier_sine(3000, 1e6),
+         "heavy_mix": lambda: atg.generate_weights("heavy_mix", 50000),
+         "tiny": lambda: np.array([1.0, 2.0]),
+         "uniform": lambda: np.ones(777)}[case]()
+    W = float(np.sum(w))
+    incl, A = oracle.rejection_build(w, W)
+    q, e = oracle.rejection_model(incl, K)
+    M = len(A) - 1
+    ks = (np.arange(K + 1, dtype=np.uint64) * np.uint64(M) + np.uint64(K - 1)) // np.uint64(K)
+    assert np.array_equal(q, np.searchsorted(incl, ks, side="right"))
+    assert np.array_equal(q, A[ks.astype(np.int64)])
+    k, pred = _model_pred(q, len(A), K, np.arange(len(A)))
+    dev = np.abs(A.astype(np.int64) - pred)
+    worst = np.zeros(K, np.int64)
+    np.maximum.at(worst, k, dev)
+    assert np.array_equal(worst, e.astype(np.int64))
+
+
 # ---- batches (test infrastructure for the engine's batched build) ------------
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
