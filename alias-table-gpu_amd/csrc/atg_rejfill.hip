@@ -58,12 +58,23 @@ __device__ __forceinline__ int32_t incl_bisect(const uint32_t* incl, int64_t lef
     return (int32_t)((uint64_t)incl[left] <= s ? right : left);
 }
 
-__device__ __forceinline__ uint64_t knot_slot(uint64_t k, uint64_t M, uint64_t K) { return (k * M + K - 1) / K; }
+// floor(num / den) for den > 0.  Below 2^53 the FP64 quotient is exact
+// enough: for a non-integer quotient q = num / den the distance to the next
+// integer is at least 1 / den, and the rounding error at most q 2^-53 < 1 /
+// den because q den = num < 2^53; an integer quotient is exact.  So the
+// floor of the correctly rounded FP64 quotient is the integer quotient, at a
+// fraction of the cost of the 64-bit integer division, which stays for
+// larger numerators.
+__device__ __forceinline__ uint64_t udiv(uint64_t num, uint64_t den) {
+    if (num < (1ull << 53)) return (uint64_t)floor((double)num / (double)den);
+    return num / den;
+}
+__device__ __forceinline__ uint64_t knot_slot(uint64_t k, uint64_t M, uint64_t K) { return udiv(k * M + K - 1, K); }
 __device__ __forceinline__ uint64_t seg_of(uint64_t s, uint64_t M, uint64_t K) {
-    return M == 0 ? 0 : min<uint64_t>(K - 1, s * K / M);
+    return M == 0 ? 0 : min<uint64_t>(K - 1, udiv(s * K, M));
 }
 __device__ __forceinline__ int64_t seg_pred(uint64_t s, uint64_t sk, uint64_t sk1, int32_t qk, int32_t qk1) {
-    return sk1 > sk ? (int64_t)qk + (int64_t)((s - sk) * (uint64_t)(qk1 - qk) / (sk1 - sk)) : (int64_t)qk;
+    return sk1 > sk ? (int64_t)qk + (int64_t)udiv((s - sk) * (uint64_t)(qk1 - qk), sk1 - sk) : (int64_t)qk;
 }
 
 // Slot s of A holds `item`; `first` is the first slot of the item's run.
@@ -135,38 +146,92 @@ __global__ __launch_bounds__(256) void rej_model_knots(RejArgs a) {
 
 // Fit, part 2: the error bounds.  Item i's run [f, l] meets segments seg(f)
 // .. seg(l); only the two end segments can deviate, each at the ends of the
-// run's part in it.  A wave whose items all lie in one segment makes one
-// atomic.
+// run's part in it.  A block takes ERR_ITEMS consecutive items.  When they
+// span at most ERR_LDS_SEGS segments, the block keeps the segments' maxima
+// in LDS (a wave whose items share one segment reduces first) and adds them
+// to the bounds with one atomic per segment: the grid walks the items in
+// order, so atomics straight to the bounds would queue on a few addresses.
+// Zero deviations are not added (the bounds start at zero).
+#define ERR_ITEMS 4096
+#define ERR_LDS_SEGS 512
 __global__ __launch_bounds__(256) void rej_model_err(RejArgs a) {
+    __shared__ uint32_t lmax[ERR_LDS_SEGS];
     if (a.st->error) return;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const bool act = i < a.n;
     const uint64_t M = a.st->rej_size - 1, K = (uint64_t)a.K;
-    uint64_t k1 = 0, k2 = 0;
-    uint32_t e1 = 0, e2 = 0;
-    if (act) {
-        const uint64_t f = i ? a.incl[i - 1] : 0, l = (uint64_t)a.incl[i] - 1;
-        k1 = seg_of(f, M, K);
-        k2 = seg_of(l, M, K);
-        auto dev = [&](uint64_t k, uint64_t s) -> uint32_t {
-            const int64_t p = seg_pred(s, knot_slot(k, M, K), knot_slot(k + 1, M, K), a.knot_items[k],
-                                       a.knot_items[k + 1]);
-            return (uint32_t)(p > i ? p - i : i - p);
-        };
-        const uint64_t end1 = (k1 == K - 1) ? M : knot_slot(k1 + 1, M, K) - 1;
-        e1 = max(dev(k1, f), dev(k1, min(l, end1)));
-        if (k2 != k1) e2 = max(dev(k2, knot_slot(k2, M, K)), dev(k2, l));
+    const int64_t i0 = (int64_t)blockIdx.x * ERR_ITEMS, i1 = min<int64_t>(a.n, i0 + ERR_ITEMS);
+    const uint64_t kb = seg_of(i0 ? a.incl[i0 - 1] : 0, M, K);
+    const uint64_t ke = seg_of((uint64_t)a.incl[i1 - 1] - 1, M, K);
+    const bool local = ke - kb < ERR_LDS_SEGS;  // the same for the whole block
+    if (local) {
+        for (uint64_t j = threadIdx.x; j <= ke - kb; j += 256) lmax[j] = 0;
+        __syncthreads();
     }
-    const uint64_t k0 = __shfl(k1, 0);
-    const uint64_t mixed = __ballot(act && (k1 != k0 || k2 != k0));
-    if (mixed == 0) {
-        uint32_t m = max(e1, e2);
+    auto add = [&](uint64_t k, uint32_t e) {
+        if (e == 0) return;
+        if (local) atomicMax(&lmax[k - kb], e);
+        else atomicMax(&a.knot_err[k], e);
+    };
+    // Wave v takes items [iw0, iw1), 1024 of the block's, in rounds of 64.
+    // When their slots all lie in one segment (the usual case), the
+    // segment's knots are read once and every item costs two predictions.
+    const int lane = threadIdx.x & 63;
+    const int64_t iw0 = min<int64_t>(i1, i0 + (int64_t)(threadIdx.x >> 6) * (ERR_ITEMS / 4));
+    const int64_t iw1 = min<int64_t>(i1, iw0 + ERR_ITEMS / 4);
+    bool one_seg = false;
+    if (iw0 < iw1) {
+        const uint64_t kw = seg_of(iw0 ? a.incl[iw0 - 1] : 0, M, K);
+        one_seg = kw == seg_of((uint64_t)a.incl[iw1 - 1] - 1, M, K);
+        if (one_seg) {
+            const uint64_t sk = knot_slot(kw, M, K), sk1 = knot_slot(kw + 1, M, K);
+            const int32_t qk = a.knot_items[kw], qk1 = a.knot_items[kw + 1];
+            uint32_t m = 0;
+            for (int64_t ib = iw0; ib < iw1; ib += 64) {
+                const int64_t i = ib + lane;
+                if (i < iw1) {
+                    const uint64_t f = i ? a.incl[i - 1] : 0, l = (uint64_t)a.incl[i] - 1;
+                    const int64_t pf = seg_pred(f, sk, sk1, qk, qk1), pl = seg_pred(l, sk, sk1, qk, qk1);
+                    m = max(m, (uint32_t)max(pf > i ? pf - i : i - pf, pl > i ? pl - i : i - pl));
+                }
+            }
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d));
-        if ((threadIdx.x & 63) == 0 && act) atomicMax(&a.knot_err[k0], m);
-    } else if (act) {
-        atomicMax(&a.knot_err[k1], e1);
-        if (k2 != k1) atomicMax(&a.knot_err[k2], e2);
+            for (int d = 1; d < 64; d <<= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d));
+            if (lane == 0) add(kw, m);
+        }
+    }
+    for (int64_t ib = iw0; !one_seg && ib < iw1; ib += 64) {
+        const int64_t i = ib + lane;
+        const bool act = i < iw1;
+        uint64_t k1 = 0, k2 = 0;
+        uint32_t e1 = 0, e2 = 0;
+        if (act) {
+            const uint64_t f = i ? a.incl[i - 1] : 0, l = (uint64_t)a.incl[i] - 1;
+            k1 = seg_of(f, M, K);
+            k2 = seg_of(l, M, K);
+            auto dev = [&](uint64_t k, uint64_t s) -> uint32_t {
+                const int64_t p = seg_pred(s, knot_slot(k, M, K), knot_slot(k + 1, M, K), a.knot_items[k],
+                                           a.knot_items[k + 1]);
+                return (uint32_t)(p > i ? p - i : i - p);
+            };
+            const uint64_t end1 = (k1 == K - 1) ? M : knot_slot(k1 + 1, M, K) - 1;
+            e1 = max(dev(k1, f), dev(k1, min(l, end1)));
+            if (k2 != k1) e2 = max(dev(k2, knot_slot(k2, M, K)), dev(k2, l));
+        }
+        const uint64_t k0 = __shfl(k1, 0);
+        const uint64_t mixed = __ballot(act && (k1 != k0 || k2 != k0));
+        if (mixed == 0) {
+            uint32_t m = max(e1, e2);
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d));
+            if ((threadIdx.x & 63) == 0 && act) add(k0, m);
+        } else if (act) {
+            add(k1, e1);
+            if (k2 != k1) add(k2, e2);
+        }
+    }
+    if (local) {
+        __syncthreads();
+        for (uint64_t j = threadIdx.x; j <= ke - kb; j += 256)
+            if (lmax[j]) atomicMax(&a.knot_err[kb + j], lmax[j]);
     }
 }
 
@@ -223,8 +288,9 @@ __global__ __launch_bounds__(FILL_THREADS) void rej_fill_pary(RejArgs a) {
 // position whose incl exceeds s0 with the one before at most s0, so a model
 // that is wrong cannot produce a wrong item, only a miss.  A miss falls back
 // to the bisection over all of incl (and is counted).  The 64 slots' items
-// lie in [a0, a0 + 63] (runs are non-empty): one coalesced load of those incl
-// and a search over the lanes by shuffles give every lane its item.
+// lie in [a0, a0 + 63] (runs are non-empty); the verifying round loads those
+// incl too, and a search over the lanes by shuffles gives every lane its
+// item.
 template <typename T>
 __global__ __launch_bounds__(FILL_THREADS) void rej_fill_learned(RejArgs a) {
     if (a.st->error) return;
@@ -254,26 +320,33 @@ __global__ __launch_bounds__(FILL_THREADS) void rej_fill_learned(RejArgs a) {
         if (m > 0) lo = jp + 1;
         hi = jm;
     }
+    // incl at position j, 0 before the first item, above every slot past the
+    // last one.
+    auto incl_at = [&](int64_t j) -> uint64_t { return j < 0 ? 0ull : (j >= n ? ~0ull : (uint64_t)a.incl[j]); };
     int64_t a0 = -1;
     uint64_t prev = 0;  // incl[a0 - 1], 0 for a0 = 0
+    uint64_t v = 0;     // lane g: incl[a0 + g]
     if (!miss) {
+        // incl[lo - 1 .. lo + 126]: the first half decides a0, and holds with
+        // the second the 64 positions from a0 on that the lanes search.
         const int64_t j = lo - 1 + lane;
-        const uint64_t v = j < 0 ? 0ull : (j >= n ? ~0ull : (uint64_t)a.incl[j]);
-        const uint64_t gt = __ballot(v > s0);
+        const uint64_t v1 = incl_at(j), v2 = incl_at(j + 64);
+        const uint64_t gt = __ballot(v1 > s0);
         const int m = gt ? __ffsll((long long)gt) - 1 : 0;
-        const uint64_t vm = __shfl(v, m > 0 ? m - 1 : 0);
         if (m > 0) {
             a0 = lo - 1 + m;
-            prev = vm;
+            prev = __shfl(v1, m - 1);
+            const int x = m + lane;
+            const uint64_t t1 = __shfl(v1, x & 63), t2 = __shfl(v2, x & 63);
+            v = x < 64 ? t1 : t2;
         }
     }
     if (a0 < 0) {
         a0 = incl_bisect(a.incl, 0, n, s0);
         prev = a0 ? a.incl[a0 - 1] : 0;
+        v = incl_at(a0 + lane);
         if (lane == 0) atomicAdd((unsigned long long*)&a.st->learned_fallbacks, 1ull);
     }
-    const int64_t jw = a0 + lane;
-    const uint64_t v = jw < n ? (uint64_t)a.incl[jw] : ~0ull;
     const uint64_t s = s0 + lane;
     // c = number of the window's incl values <= s (they increase; the last
     // exceeds s0 + 63), so the item of s is a0 + c.
@@ -293,7 +366,7 @@ hipError_t launch_rejection_incl(const RejArgs& a, bool model, hipStream_t s) {
     else hipLaunchKernelGGL(rej_incl<double>, dim3(nb), dim3(REJ_THREADS), 0, s, a);
     if (model) {
         hipLaunchKernelGGL(rej_model_knots, dim3((unsigned)((a.K + 1 + 255) / 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(rej_model_err, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(rej_model_err, dim3((unsigned)((a.n + ERR_ITEMS - 1) / ERR_ITEMS)), dim3(256), 0, s, a);
     }
     return hipGetLastError();
 }

@@ -178,6 +178,7 @@ __global__ __launch_bounds__(SECTIONED_THREADS) void sample_sectioned(SampleArgs
 // scanning the totals, then each block writes its items' slots of A; the
 // slots of long runs are cut into tasks that the whole grid writes.
 #define REJ_WAVE 64     // runs up to this are written by their wave, longer ones as tasks
+#define REJ_WAVE_BUDGET 8192  // ... in waves with more slots than this; other waves write all their runs
 #define REJ_TASK 4096   // slots per task
 #define REJ_LONG_BLOCKS 1024
 
@@ -272,13 +273,16 @@ __global__ __launch_bounds__(1024) void rej_scan_totals(RejArgs a) {
 // carries the item's acceptance fraction (FP64; the reference computes it in
 // float, RejectionSampling.cu:26-28), the others accept always.  The wave
 // writes the runs of a round together, consecutive slots by consecutive
-// lanes; the rest of a run longer than REJ_WAVE slots becomes tasks of
-// REJ_TASK slots for rej_fill_long, so that heavy items, however clustered,
-// hold up no wave for long.
+// lanes.  In a wave with more than REJ_WAVE_BUDGET slots, the rest of a run
+// longer than REJ_WAVE slots becomes tasks of REJ_TASK slots for
+// rej_fill_long, so that heavy items, however clustered, hold up no wave for
+// long; a wave within the budget writes every run itself.
 template <typename T>
 __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
     constexpr int U = REJ_ITEMS / REJ_THREADS;
     __shared__ uint64_t wtot[REJ_THREADS / 64];
+    __shared__ uint32_t wtask[REJ_THREADS / 64];
+    __shared__ uint32_t task_base;
     if (a.st->error) return;
     const double W = a.st->W;
     const T* w = (const T*)a.weights;
@@ -300,10 +304,32 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
         total += c[u];
     }
     total = wave_sum_int<uint64_t>(total);
-    if (lane == 0) wtot[wave] = total;
+    // A wave with at most REJ_WAVE_BUDGET slots writes all its runs; a wave
+    // with more hands the runs longer than REJ_WAVE to tasks.  The tasks are
+    // placed with one atomic per block (a counter bumped per round would
+    // serialise a grid of waves on it).
+    const uint32_t long_min = total > REJ_WAVE_BUDGET ? (uint32_t)REJ_WAVE : 0xFFFFFFFFu;
+    uint32_t ntask = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        ntask += (c[u] > long_min) ? (c[u] - 1 + REJ_TASK - 1) / REJ_TASK : 0u;
+    ntask = wave_sum_int<uint32_t>(ntask);
+    if (lane == 0) {
+        wtot[wave] = total;
+        wtask[wave] = ntask;
+    }
     __syncthreads();
     uint64_t first0 = a.block_off[blockIdx.x];
     for (int q = 0; q < wave; ++q) first0 += wtot[q];
+    uint32_t block_tasks = 0;
+    for (int q = 0; q < REJ_THREADS / 64; ++q) block_tasks += wtask[q];
+    uint32_t t_run = 0;  // the wave's next task
+    if (block_tasks != 0) {
+        if (threadIdx.x == 0) task_base = atomicAdd(&a.st->rej_tasks, block_tasks);
+        __syncthreads();
+        t_run = task_base;
+        for (int q = 0; q < wave; ++q) t_run += wtask[q];
+    }
     for (int u = 0; u < U; ++u) {
         const int k = u * 64 + lane;
         // A round's counts sum to at most |A| <= 2n < 2^32: 32-bit scans.
@@ -317,7 +343,7 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
         // The round's slots in the wave, long runs counted by their first slot
         // only: lane v of each pass writes the v-th of them, owned by the
         // last item whose local start is <= v (a search over the 64 starts).
-        const bool long_run = k < cnt && cu > REJ_WAVE;
+        const bool long_run = k < cnt && cu > long_min;
         const uint64_t any_long = __ballot(long_run);
         uint32_t lex = incl - cu, L = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         if (any_long != 0ull) {
@@ -353,9 +379,8 @@ __global__ __launch_bounds__(REJ_THREADS) void rej_fill(RejArgs a) {
             const uint32_t ti = wave_incl_scan_u32(nt);
             const uint32_t tex = ti - nt;
             const uint32_t tsum = (uint32_t)__builtin_amdgcn_readlane((int)ti, 63);
-            uint32_t t0 = 0;
-            if (lane == 0) t0 = atomicAdd(&a.st->rej_tasks, tsum);
-            t0 = __shfl(t0, 0) + tex;
+            const uint32_t t0 = t_run + tex;
+            t_run += tsum;
             if (long_run) {
                 if ((uint64_t)t0 + nt > a.tasks_cap) {
                     atomicOr(&a.st->error, ERR_BAD_SUM);
