@@ -63,6 +63,12 @@ def _split_id(split):
     return _lib.SPLIT_METHODS[split]
 
 
+def _pack_id(pack):
+    if pack not in _lib.PACK_METHODS:
+        raise AtgError(_EINVAL, f"unknown pack method {pack!r}; one of {sorted(_lib.PACK_METHODS)}")
+    return _lib.PACK_METHODS[pack]
+
+
 def _split_name(method):
     return {v: k for k, v in _lib.SPLIT_METHODS.items()}[int(method)]
 
@@ -153,7 +159,8 @@ def autotune_shape(w, dtype="f64", layout="aos", psa_plus=True, split="bisect", 
 
 
 def build_noalloc(w, rows, aliases=None, W_out=None, error_out=None, dtype="f64", layout="aos", psa_plus=True,
-                  chains=16, section_size=0, num_splits=0, split="bisect", learned_segments=4096, stream=None):
+                  chains=16, section_size=0, num_splits=0, split="bisect", learned_segments=4096, stream=None,
+                  pack="chunked"):
     """Builds the table of the device weights `w` into caller-owned device
     buffers with no allocation (atg_build_noalloc): AoS rows as a float64
     tensor of 2 n elements (16-byte rows), or SoA row weights (float64, n) and
@@ -171,6 +178,7 @@ def build_noalloc(w, rows, aliases=None, W_out=None, error_out=None, dtype="f64"
     _device_buffer(W_out, "f64", 1, "build_noalloc: W_out")
     _device_buffer(error_out, "i32", 1, "build_noalloc: error_out")
     o = _shape_opts(psa_plus, chains, section_size, num_splits, split, learned_segments)
+    o.pack_method = _pack_id(pack)
     check(lib().atg_build_noalloc(_ptr(w), ATG_F32 if dtype == "f32" else ATG_F64, int(n),
                                   ATG_SOA if layout == "soa" else ATG_AOS, C.byref(o), _ptr(rows), _ptr(aliases),
                                   _ptr(W_out), _ptr(error_out), _stream(stream)))
@@ -206,9 +214,10 @@ class Table:
     """One alias table on the GPU (AliasTableSplitGpu<TableStorage> of the reference)."""
 
     def __init__(self, n, dtype="f64", layout="aos", psa_plus=True, chains=16, section_size=0, num_splits=0,
-                 split="bisect", learned_segments=4096, record_phases=False, interleave=False):
+                 split="bisect", learned_segments=4096, record_phases=False, interleave=False, pack="chunked"):
         o = default_opts()
         o.interleave = 1 if interleave else 0
+        o.pack_method = _pack_id(pack)
         o.psa_plus = 1 if psa_plus else 0
         o.chains = int(chains)
         o.section_size = int(section_size)

@@ -18,6 +18,8 @@ ATG_MEM_HOST, ATG_MEM_DEVICE = 0, 1
 ATG_SPLIT_BISECT, ATG_SPLIT_LEARNED, ATG_SPLIT_PARY = 0, 1, 2
 SPLIT_METHODS = {"bisect": ATG_SPLIT_BISECT, "learned": ATG_SPLIT_LEARNED, "pary": ATG_SPLIT_PARY}
 ATG_SAMPLE_BASIC, ATG_SAMPLE_SECTIONED = 0, 1
+ATG_PACK_CHUNKED, ATG_PACK_BASIC, ATG_PACK_NO_WEIGHTS = 0, 1, 2
+PACK_METHODS = {"chunked": ATG_PACK_CHUNKED, "basic": ATG_PACK_BASIC, "no_weights": ATG_PACK_NO_WEIGHTS}
 ATG_FILL_DIRECT, ATG_FILL_BISECT, ATG_FILL_PARY, ATG_FILL_LEARNED = 0, 1, 2, 3
 FILL_METHODS = {"direct": ATG_FILL_DIRECT, "bisect": ATG_FILL_BISECT, "pary": ATG_FILL_PARY,
                 "learned": ATG_FILL_LEARNED}
@@ -32,7 +34,8 @@ DIST = {
 class BuildOpts(C.Structure):
     _fields_ = [("psa_plus", C.c_int32), ("chains", C.c_int32), ("section_size", C.c_int64),
                 ("num_splits", C.c_int64), ("split_method", C.c_int32), ("learned_segments", C.c_int32),
-                ("record_phases", C.c_int32), ("interleave", C.c_int32), ("batch_group_items", C.c_int64)]
+                ("record_phases", C.c_int32), ("interleave", C.c_int32), ("batch_group_items", C.c_int64),
+                ("pack_method", C.c_int32), ("reserved0", C.c_int32)]
 
 
 class Stats(C.Structure):

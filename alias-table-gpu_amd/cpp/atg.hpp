@@ -81,6 +81,10 @@ struct Variant {
     // performSplitAndPackInterleaved (AliasTableSplitGpu.cu:220-262): split
     // and pack on two streams; no per-phase timers then.
     bool interleave = false;
+    // Variant::pack: CHUNKED (LDS-staged), BASIC or NO_WEIGHTS.  As in
+    // Variant::setPsaPlusIfSupported (AliasTableSplitGpu.cuh:86-93), PSA+ is
+    // off with NO_WEIGHTS.
+    atg_pack_method pack = ATG_PACK_CHUNKED;
 };
 
 template <Storage S = Storage::ArrayOfStructs>
@@ -96,8 +100,10 @@ class AliasTableSplitGpu : public SamplingAlgorithm {
         std::string v = variant_.split == ATG_SPLIT_LEARNED ? "LearnedSplit"
                         : variant_.split == ATG_SPLIT_PARY  ? "ParySplit"
                                                             : "";
+        if (variant_.pack == ATG_PACK_BASIC) v += "PackBasic";
+        if (variant_.pack == ATG_PACK_NO_WEIGHTS) v += "PackWithoutWeights";
         if (variant_.interleave) v += "Interleaved";
-        if (variant_.psaPlus) v += "PsaPlus";
+        if (variant_.psaPlus && variant_.pack != ATG_PACK_NO_WEIGHTS) v += "PsaPlus";
         return std::string("AliasTableSplitGpu") +
                (S == Storage::ArrayOfStructs ? "ArrayOfStructs" : "StructOfArrays") + v;
     }
@@ -107,7 +113,8 @@ class AliasTableSplitGpu : public SamplingAlgorithm {
         if (!table_) {
             atg_build_opts o;
             atg_default_opts(&o);
-            o.psa_plus = variant_.psaPlus ? 1 : 0;
+            o.psa_plus = variant_.psaPlus && variant_.pack != ATG_PACK_NO_WEIGHTS ? 1 : 0;
+            o.pack_method = variant_.pack;
             o.chains = variant_.chains;
             o.section_size = variant_.sectionSize;
             o.num_splits = numThreads_ == NUMBER_THREADS_AUTO ? 0 : numThreads_;

@@ -49,6 +49,15 @@ static void testTableCorrectness() {
             }
         }
     }
+    // The pack variants (Variant::pack) on every distribution.
+    for (int d = 1; d <= MAX_WEIGHT_DISTRIBUTION; ++d) {
+        for (atg_pack_method pm : {ATG_PACK_BASIC, ATG_PACK_NO_WEIGHTS}) {
+            Variant v;
+            v.pack = pm;
+            expect_build<Storage::ArrayOfStructs>(100000, v, NUMBER_THREADS_AUTO, (WeightDistribution)d);
+            expect_build<Storage::StructOfArrays>(10000, v, 1024, (WeightDistribution)d);
+        }
+    }
     // Regression: a section group with zero eligible pack lanes (UnitTests.cu:47-49).
     Variant v;
     v.psaPlus = false;

@@ -149,9 +149,14 @@ struct atg_table_s {
         a.pack_cap = pack_cap_items(n, p_req);
         a.learned_segments = learned_knots(opts.learned_segments, p_req);
         a.knot_j = d_knots;
+        a.pack_method = opts.pack_method;
         return a;
     }
 };
+
+static_assert((int)PACK_CHUNKED == (int)ATG_PACK_CHUNKED && (int)PACK_BASIC == (int)ATG_PACK_BASIC &&
+                  (int)PACK_NO_WEIGHTS == (int)ATG_PACK_NO_WEIGHTS,
+              "pack method values");
 
 extern "C" {
 
@@ -211,6 +216,11 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     if (o.chains < 1 || o.chains > 64) return fail(ATG_EINVAL, "atg_create: chains=%d outside [1, 64]", o.chains);
     if (o.split_method != ATG_SPLIT_BISECT && o.split_method != ATG_SPLIT_LEARNED && o.split_method != ATG_SPLIT_PARY)
         return fail(ATG_EINVAL, "atg_create: unknown split method %d", o.split_method);
+    if (o.pack_method != ATG_PACK_CHUNKED && o.pack_method != ATG_PACK_BASIC && o.pack_method != ATG_PACK_NO_WEIGHTS)
+        return fail(ATG_EINVAL, "atg_create: unknown pack method %d", o.pack_method);
+    if (o.pack_method == ATG_PACK_NO_WEIGHTS && o.psa_plus)
+        return fail(ATG_EINVAL, "atg_create: the NO_WEIGHTS pack needs psa_plus = 0 (PSA+ leaves heavy residues "
+                                "that differ from the input weights)");
     if (o.learned_segments <= 0) o.learned_segments = 4096;
     if (o.section_size <= 0) o.section_size = std::min<int64_t>(1000, n / 10);
     if (o.section_size < 1) o.section_size = 1;

@@ -462,6 +462,8 @@ atg_status atg_batch_create(atg_batch** out, const int64_t* offsets, int64_t nta
         return fail(ATG_ENOTSUP, "atg_batch_create: the P-ary split is built for single tables; use bisection or the learned split");
     if (o.split_method != ATG_SPLIT_BISECT && o.split_method != ATG_SPLIT_LEARNED)
         return fail(ATG_EINVAL, "atg_batch_create: unknown split method %d", o.split_method);
+    if (o.pack_method != ATG_PACK_CHUNKED)
+        return fail(ATG_ENOTSUP, "atg_batch_create: the batch packs with the staged (chunked) pack only");
     if (o.learned_segments <= 0) o.learned_segments = 4096;
     const int64_t p_auto = 32LL * device_cus() * 10;
 

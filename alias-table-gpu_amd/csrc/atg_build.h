@@ -9,6 +9,9 @@
 namespace atg {
 
 // Everything a build kernel needs, passed by value (all device pointers).
+// Pack methods (the values of atg_pack_method).
+enum PackMethod : int32_t { PACK_CHUNKED = 0, PACK_BASIC = 1, PACK_NO_WEIGHTS = 2 };
+
 struct BuildArgs {
     const void* weights;  // n weights (double or float)
     int dtype_f32;
@@ -38,6 +41,7 @@ struct BuildArgs {
     // learned split index
     int32_t learned_segments;
     int32_t* knot_j;          // learned_segments + 1 exact split j at the knots
+    int32_t pack_method;      // atg_pack_method
 };
 
 // Section layout of PSA+ (AliasTableSplitGpu.cu:356-357, PsaPlusNormal.cu:27-29).

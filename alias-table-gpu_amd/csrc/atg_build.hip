@@ -97,6 +97,26 @@ __global__ __launch_bounds__(64) void pack_staged(BuildArgs a, int64_t blk0) {
     pack_staged_block<AOS>(a, blk0 + blockIdx.x, lds_raw);
 }
 
+// 3d. Pack with one lane per chain and the lists read from HBM
+//     (PackMethodBasic::packKernel, PackMethodBasic.cu:60-73), or with the
+//     weights read through the items (PackMethodNoWeights::packKernel,
+//     PackMethodNoWeights.cu:136-145).  Blocks of 64 chains, as pack_staged.
+template <bool AOS, bool NOW, typename T>
+__global__ __launch_bounds__(64) void pack_basic(BuildArgs a, int64_t blk0) {
+    const DevState* st = a.st;
+    if (st->error || st->degenerate) return;
+    const int64_t k = (blk0 + blockIdx.x) * 64 + threadIdx.x;  // chain k + 1
+    if (k + 1 > st->p) return;
+    const Split prev = a.splits[k], cur = a.splits[k + 1];
+    if (NOW) {
+        const NoWeightsLists<T> l{a.Li, a.Hi, (const T*)a.weights, a.n};
+        pack_chain<AOS, false>(a, l, prev, cur, st->W_N);
+    } else {
+        const GlobalLists l{a.Li, a.Hi, a.Lw, a.Hw};
+        pack_chain<AOS, false>(a, l, prev, cur, st->W_N);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers.
 static inline unsigned blocks_for(int64_t items, int per) { return (unsigned)((items + per - 1) / per); }
@@ -181,8 +201,22 @@ hipError_t launch_split(const BuildArgs& a, hipStream_t s) {
 hipError_t launch_pack_blocks(const BuildArgs& a, hipStream_t s, int64_t b0, int64_t b1) {
     const size_t lds = pack_lds_bytes(a.pack_cap);
     if (b1 <= b0) return hipSuccess;
-    if (a.rows) hipLaunchKernelGGL(pack_staged<true>, dim3((unsigned)(b1 - b0)), dim3(64), lds, s, a, b0);
-    else hipLaunchKernelGGL(pack_staged<false>, dim3((unsigned)(b1 - b0)), dim3(64), lds, s, a, b0);
+    const dim3 g((unsigned)(b1 - b0)), b(64);
+    if (a.pack_method == PACK_BASIC) {
+        if (a.rows) hipLaunchKernelGGL((pack_basic<true, false, double>), g, b, 0, s, a, b0);
+        else hipLaunchKernelGGL((pack_basic<false, false, double>), g, b, 0, s, a, b0);
+    } else if (a.pack_method == PACK_NO_WEIGHTS) {
+        if (a.dtype_f32) {
+            if (a.rows) hipLaunchKernelGGL((pack_basic<true, true, float>), g, b, 0, s, a, b0);
+            else hipLaunchKernelGGL((pack_basic<false, true, float>), g, b, 0, s, a, b0);
+        } else {
+            if (a.rows) hipLaunchKernelGGL((pack_basic<true, true, double>), g, b, 0, s, a, b0);
+            else hipLaunchKernelGGL((pack_basic<false, true, double>), g, b, 0, s, a, b0);
+        }
+    } else {
+        if (a.rows) hipLaunchKernelGGL(pack_staged<true>, g, b, lds, s, a, b0);
+        else hipLaunchKernelGGL(pack_staged<false>, g, b, lds, s, a, b0);
+    }
     return hipGetLastError();
 }
 

@@ -617,6 +617,27 @@ struct GlobalLists {
     __device__ int hi(int j) const { return Hi[j]; }
     __device__ double hw(int j) const { return Hw[j]; }
 };
+// The lists without their weight arrays: each weight read through its item
+// from the input weights (PackMethodNoWeights.cu:3-60 reads them from a table
+// prefilled with the weights).  Without PSA+ the lists' weights are the input
+// weights, so the chains see the same values; the sentinels keep theirs (the
+// light one, item n + 1, weighs 0; the heavy one, item n, +inf).
+template <typename T>
+struct NoWeightsLists {
+    const int32_t *Li, *Hi;
+    const T* w;
+    int64_t n;
+    __device__ int li(int i) const { return Li[i]; }
+    __device__ double lw(int i) const {
+        const int it = Li[i];
+        return (uint32_t)it < (uint32_t)n ? (double)w[it] : 0.0;
+    }
+    __device__ int hi(int j) const { return Hi[j]; }
+    __device__ double hw(int j) const {
+        const int it = Hi[j];
+        return (uint32_t)it < (uint32_t)n ? (double)w[it] : __builtin_inf();
+    }
+};
 struct StagedLists {
     const int32_t *Li, *Hi;
     const double *Lw, *Hw;

@@ -86,6 +86,10 @@ extern "C" atg_status atg_build_noalloc(const void* weights, atg_dtype dtype, in
     if (opts) o = *opts; else atg_default_opts(&o);
     if (o.interleave || o.record_phases)
         return fail(ATG_ENOTSUP, "atg_build_noalloc: interleave and record_phases need streams and events of their own");
+    if (o.pack_method != ATG_PACK_CHUNKED && o.pack_method != ATG_PACK_BASIC && o.pack_method != ATG_PACK_NO_WEIGHTS)
+        return fail(ATG_EINVAL, "atg_build_noalloc: unknown pack method %d", o.pack_method);
+    if (o.pack_method == ATG_PACK_NO_WEIGHTS && o.psa_plus)
+        return fail(ATG_EINVAL, "atg_build_noalloc: the NO_WEIGHTS pack needs psa_plus = 0");
     if (o.chains < 1 || o.chains > 64) return fail(ATG_EINVAL, "atg_build_noalloc: chains=%d outside [1, 64]", o.chains);
     if (o.split_method != ATG_SPLIT_BISECT && o.split_method != ATG_SPLIT_LEARNED && o.split_method != ATG_SPLIT_PARY)
         return fail(ATG_EINVAL, "atg_build_noalloc: unknown split method %d", o.split_method);
@@ -146,6 +150,7 @@ extern "C" atg_status atg_build_noalloc(const void* weights, atg_dtype dtype, in
     a.p_req = p_req;
     a.pack_cap = pack_cap_items(n, p_req);
     a.learned_segments = learned_knots(o.learned_segments, p_req);
+    a.pack_method = o.pack_method;
 
     hipStream_t s = (hipStream_t)stream;
     HIPCHK(set_partition_lds_limit(psa_lds_bytes(Ls)));

@@ -45,6 +45,19 @@ typedef enum {
                                probe 16 positions per step; bisection fallback; single tables only */
 } atg_split_method;
 
+/* Pack of the leftover lists into rows (Variant::pack and sharedMemory,
+ * AliasTableSplitGpu.cuh:38-94, dispatched in performPack,
+ * AliasTableSplitGpu.cu:506-591).  Every method writes the same rows. */
+typedef enum {
+    ATG_PACK_CHUNKED = 0,    /* one wave per 64 chains, their lists staged in LDS in rounds
+                                (the role of PackMethodBasicShared / PackMethodChunkedShared); default */
+    ATG_PACK_BASIC = 1,      /* one lane per chain, lists read from HBM (PackMethodBasic::packKernel) */
+    ATG_PACK_NO_WEIGHTS = 2  /* as BASIC, each weight read through its item from the input weights
+                                instead of the lists (PackMethodNoWeights::packKernel); needs
+                                psa_plus = 0 (the reference turns PSA+ off for it,
+                                AliasTableSplitGpu.cuh:86-93); single tables only */
+} atg_pack_method;
+
 typedef enum {
     ATG_SAMPLE_BASIC = 0,      /* SamplerGpuBasic (SamplerGpuBasic.cu:77-133) */
     ATG_SAMPLE_SECTIONED = 1   /* SamplerGpuSectioned(Shared) (SamplerGpuSectioned.cu:98-166) */
@@ -72,6 +85,8 @@ typedef struct {
                                 phases run group after group, so a table's build latency is its
                                 group's, not the whole batch's; <= 0: one group.  Same tables
                                 either way. */
+    int32_t pack_method;     /* atg_pack_method (Variant::pack + sharedMemory); same table for each */
+    int32_t reserved0;       /* 0 */
 } atg_build_opts;
 
 typedef struct {

@@ -254,3 +254,37 @@ def test_post_build_rejects_corrupted_table(gpu, oracle, corrupt):
     t.set_table(rw, ra, t.stats()["W"])
     rel, ok = t.post_build()
     assert not ok and rel > 1e-3
+
+
+@pytest.mark.parametrize("pack", ["basic", "no_weights"])
+@pytest.mark.parametrize("n,dist,psa,p,split,layout,dtype", [
+    (100003, "pareto", False, 1400, "bisect", "aos", "f64"),
+    (100003, "sine", False, 0, "learned", "soa", "f64"),
+    (1 << 20, "heavy_mix", False, 16384, "bisect", "aos", "f32"),
+    (505711, "one_light", False, 16576, "bisect", "aos", "f64"),
+    (65536, "pareto", True, 0, "bisect", "aos", "f64"),
+    (11, "unit_uniform", False, 0, "bisect", "soa", "f64"),
+])
+def test_pack_methods_identical(gpu, oracle, pack, n, dist, psa, p, split, layout, dtype):
+    """The pack variants of Variant::pack (PackMethodBasic, PackMethodNoWeights)
+    write the oracle's rows, as the default LDS-staged pack does; the
+    NO_WEIGHTS pack refuses PSA+ (whose heavy residues are not input
+    weights)."""
+    w = atg.generate_weights(dist, n, seed=17, param=1.1 if dist == "pareto" else 0.9)
+    if dtype == "f32":
+        w = w.astype(np.float32)
+    if pack == "no_weights" and psa:
+        with pytest.raises(atg.AtgError) as e:
+            atg.Table(n, psa_plus=True, pack=pack)
+        assert e.value.status == 1
+        return
+    kw = dict(psa_plus=psa, num_splits=p, split=split, layout=layout)
+    t = gpu_build(w, dtype=dtype, pack=pack, **kw)
+    ref = oracle.build(w.astype(np.float64), psa_plus=psa,
+                       num_splits=p or 320 * gpu.cuda.get_device_properties(0).multi_processor_count,
+                       want_lists=False)
+    assert_same_build(t, ref, n, check_lists=False)
+    t.build()  # a rebuild on the same handle
+    t.sync()
+    assert_same_build(t, ref, n, check_lists=False)
+    t.close()

@@ -84,3 +84,10 @@ def test_noalloc_rejects_beyond_scratch_and_unsupported(gpu):
     st = atg.lib().atg_build_noalloc(small.data_ptr(), 0, 1000, 0, C.byref(o), rows.data_ptr(), None, None, None,
                                      None)
     assert st == 5  # ATG_ENOTSUP
+
+
+@pytest.mark.parametrize("pack,psa", [("basic", True), ("no_weights", False)])
+def test_noalloc_pack_methods(gpu, pack, psa):
+    torch = gpu
+    check_against_build(torch, 100_003, "pareto", "aos", "f64", "bisect", 5, pack=pack, psa_plus=psa)
+    check_against_build(torch, 5000, "sine", "soa", "f32", "learned", 6, pack=pack, psa_plus=psa)
