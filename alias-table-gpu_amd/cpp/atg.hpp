@@ -82,7 +82,7 @@ struct Variant {
     // and pack on two streams; no per-phase timers then.
     bool interleave = false;
     // Variant::pack: CHUNKED (LDS-staged), BASIC or NO_WEIGHTS.  As in
-    // Variant::setPsaPlusIfSupported (AliasTableSplitGpu.cuh:86-93), PSA+ is
+    // Variant::setPsaPlusIfSupported (AliasTableSplitGpu.cuh:85-93), PSA+ is
     // off with NO_WEIGHTS.
     atg_pack_method pack = ATG_PACK_CHUNKED;
 };

@@ -10,11 +10,14 @@ unchanged (split method, knots, split/pack interleave) tuned at run time.
 One step = one whole build over the resident weights.
 
 Also measured: 1e9 sectioned samples (4096-row sections), basic sampling and
-the rejection variant on that table; the learned split against bisection on
-N = 1e8 heavy-mix weights; the two batched workloads (65,536 FP64 tables and
-16,384 FP32 tables with the learned split) with graph replay and per-table
-latencies; the implied-probability error of the tables; the build at the
-reference's default shape; and the CPU baselines on the host's cores.
+the rejection variant on that table; the four fills of the rejection array
+(direct, bisection, p-ary, learned inverse-CDF index) on the Pareto and the
+heavy-mix weights; the pack variants at the headline shape; the learned and
+the p-ary split against bisection on N = 1e8 heavy-mix weights; the two
+batched workloads (65,536 FP64 tables and 16,384 FP32 tables with the learned
+split) with graph replay and per-table latencies; the implied-probability
+error of the tables; the build at the reference's default shape; and the CPU
+baselines on the host's cores.
 
 Prints one JSON line.  `value` is build throughput (Gitems/s, weights already
 in HBM); `roofline` is the dominant build kernel's algorithmic bytes over its
@@ -183,6 +186,27 @@ def bench_reference_shape(args, stream, w):
     torch.cuda.empty_cache()
     return {"section_size": st["section_len"], "chains": 16, "p": st["p"], "split": "bisect",
             "build_ms": round(ms, 4), "gitems_s": round(args.n / (ms * 1e-3) / 1e9, 3), "R_over_N": round(st["R"] / args.n, 4)}
+
+
+def bench_pack_methods(args, stream, w, shape):
+    """The pack variants at the headline shape: whole builds and pack phase
+    times, the LDS-staged default against one lane per chain from HBM (PSA+
+    on), and both against the weights-through-items pack (PSA+ off)."""
+    import torch
+    import atg
+    res = {}
+    for name, pack, psa in (("chunked", "chunked", True), ("basic", "basic", True),
+                            ("chunked_no_psa", "chunked", False), ("no_weights_no_psa", "no_weights", False)):
+        t = atg.Table(args.n, pack=pack, psa_plus=psa, record_phases=True, **shape)
+        t.set_weights(w, stream)
+        t.build(stream)
+        torch.cuda.synchronize()
+        ms, _ = time_events(stream, lambda: t.build(stream), max(3, args.steps // 4))
+        res[name] = {"build_ms": round(ms, 4), "pack_ms": round(t.stats()["phase_ms"][4], 4)}
+        t.close()
+    torch.cuda.empty_cache()
+    log(f"[bench] pack variants: {res}")
+    return res
 
 
 def bench_learned(args, stream):
@@ -413,6 +437,7 @@ def main():
     ap.add_argument("--learned-chains", type=int, default=12)
     ap.add_argument("--sample-section", type=int, default=4096)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-pack-methods", action="store_true", help="skip the pack variants")
     ap.add_argument("--no-rejection-fills", action="store_true",
                     help="skip the fills of the rejection array (direct, bisection, p-ary, learned)")
     ap.add_argument("--no-batch", action="store_true")
@@ -509,6 +534,8 @@ def main():
     if not args.no_rejection_fills:
         result["rejection_fills"] = bench_rejection_fills(args, stream, w, f"N={n:.0e} Pareto(1.1)")
     torch.cuda.empty_cache()
+    if not args.no_pack_methods:
+        result["pack_methods"] = bench_pack_methods(args, stream, w, shape)
     if not args.no_reference_shape:
         result["reference_shape"] = bench_reference_shape(args, stream, w)
         log(f"[bench] reference shape: {result['reference_shape']}")

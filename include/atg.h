@@ -55,7 +55,7 @@ typedef enum {
     ATG_PACK_NO_WEIGHTS = 2  /* as BASIC, each weight read through its item from the input weights
                                 instead of the lists (PackMethodNoWeights::packKernel); needs
                                 psa_plus = 0 (the reference turns PSA+ off for it,
-                                AliasTableSplitGpu.cuh:86-93); single tables only */
+                                AliasTableSplitGpu.cuh:85-93); single tables only */
 } atg_pack_method;
 
 typedef enum {
