@@ -69,6 +69,9 @@ struct atg_table_s {
     double* d_tree = nullptr;
     uint64_t* d_slots = nullptr;
     uint64_t* d_sec_cert = nullptr;
+    uint32_t* d_sec_loff = nullptr;  // lean light lists: lights before each section
+    uint32_t* d_sec_ldir = nullptr;  // lean light lists: the section of every 1024th light
+    bool placed = false;             // the staging and sec_loff of the last partition are current
     double* d_stage_w = nullptr;
     int32_t* d_stage_i = nullptr;
     int32_t *d_Li = nullptr, *d_Hi = nullptr;
@@ -150,6 +153,9 @@ struct atg_table_s {
         a.learned_segments = learned_knots(opts.learned_segments, p_req);
         a.knot_j = d_knots;
         a.pack_method = opts.pack_method;
+        a.sec_loff = d_sec_loff;
+        a.sec_ldir = d_sec_ldir;
+        a.lean_lights = d_sec_loff && d_sec_ldir && lean_lights_ok(n, p_req, opts.pack_method) ? 1 : 0;
         return a;
     }
 };
@@ -182,7 +188,7 @@ int32_t atg_device_count(void) {
 
 static void free_table(atg_table* t) {
     if (!t) return;
-    void* ptrs[] = {t->d_weights_owned, t->d_state, t->d_section_sums, t->d_tree, t->d_slots, t->d_sec_cert, t->d_stage_w,
+    void* ptrs[] = {t->d_weights_owned, t->d_state, t->d_section_sums, t->d_tree, t->d_slots, t->d_sec_cert, t->d_sec_loff, t->d_sec_ldir, t->d_stage_w,
                     t->d_stage_i, t->d_Li, t->d_Hi,
                     t->d_Lw, t->d_Lp, t->d_Hw, t->d_Hp, t->d_rows, t->d_tw, t->d_ta, t->d_splits, t->d_knots,
                     t->d_checksum, t->d_rej_state, t->d_A, t->d_rej_tot, t->d_rej_off, t->d_rej_tasks,
@@ -262,6 +268,8 @@ atg_status atg_create(atg_table** out, int64_t n, atg_dtype dtype, atg_layout la
     ALLOC(t->d_Hp, n + 2);
     ALLOC(t->d_stage_w, n);
     ALLOC(t->d_sec_cert, 6 * t->G);
+    ALLOC(t->d_sec_loff, t->G + 1);
+    ALLOC(t->d_sec_ldir, light_dir_size(n));
     ALLOC(t->d_stage_i, n);
     if (layout == ATG_AOS) {
         ALLOC(t->d_rows, n);
@@ -323,6 +331,7 @@ atg_status atg_set_weights(atg_table* t, const void* w, atg_mem mem, void* strea
     }
     t->weights_set = true;
     t->built = false;
+    t->placed = false;
     t->rej_built = false;
     t->last_stream = s;
     return ATG_OK;
@@ -336,9 +345,18 @@ atg_status atg_build_phase(atg_table* t, int32_t phase, void* stream) {
     const BuildArgs a = t->args();
     switch (phase) {
         case ATG_PHASE_SUM: HIPCHK(launch_sum(a, s)); break;
-        case ATG_PHASE_PARTITION: HIPCHK(launch_partition(a, s)); break;
-        case ATG_PHASE_PARTITION_LOCAL: HIPCHK(launch_partition_local(a, s)); break;
-        case ATG_PHASE_PARTITION_PLACE: HIPCHK(launch_partition_place(a, s)); break;
+        case ATG_PHASE_PARTITION:
+            HIPCHK(launch_partition(a, s));
+            t->placed = true;
+            break;
+        case ATG_PHASE_PARTITION_LOCAL:
+            t->placed = false;
+            HIPCHK(launch_partition_local(a, s));
+            break;
+        case ATG_PHASE_PARTITION_PLACE:
+            HIPCHK(launch_partition_place(a, s));
+            t->placed = true;
+            break;
         case ATG_PHASE_SPLIT:
             if (t->opts.split_method == ATG_SPLIT_LEARNED) HIPCHK(launch_split_learned(a, s));
             else if (t->opts.split_method == ATG_SPLIT_PARY) HIPCHK(launch_split_pary(a, s));
@@ -393,6 +411,7 @@ atg_status atg_build(atg_table* t, void* stream) {
         const BuildArgs a = t->args();
         HIPCHK(launch_sum(a, s));
         HIPCHK(launch_partition(a, s));
+        t->placed = true;
         hipError_t err = hipSuccess;
         if (build_split_pack_interleaved(t, a, s, err)) {
             HIPCHK(err);
@@ -534,6 +553,7 @@ atg_status atg_set_table(atg_table* t, const double* w, const int32_t* al, doubl
     HIPCHK(hipMemcpyAsync(t->d_state, &st, sizeof(DevState), hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
     t->built = true;
+    t->placed = false;
     t->rej_built = false;
     t->last_stream = s;
     return ATG_OK;
@@ -570,6 +590,11 @@ atg_status atg_get_splits(atg_table* t, int32_t* i, int32_t* j, double* spill, i
 atg_status atg_get_lh(atg_table* t, int32_t* Li, double* Lw, double* Lp, int32_t* Hi, double* Hw, double* Hp) {
     if (!t) return fail(ATG_EINVAL, "atg_get_lh: NULL table");
     DevState st;
+    const BuildArgs a = t->args();
+    if (a.lean_lights && t->placed && (Li || Lw)) {
+        // The light items and weights stay in the staging: write them out as lists first.
+        HIPCHK(launch_lean_lights_fill(a, t->last_stream));
+    }
     HIPCHK(hipStreamSynchronize(t->last_stream));
     HIPCHK(hipMemcpy(&st, t->d_state, sizeof(st), hipMemcpyDeviceToHost));
     if (Li) HIPCHK(hipMemcpy(Li, t->d_Li, 4 * st.numL, hipMemcpyDeviceToHost));

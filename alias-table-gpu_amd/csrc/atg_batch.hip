@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void seg_place(SegArgs g, SegMap m) {
     if (a.st->error) return;
     const int64_t s = blk * 4 + (threadIdx.x >> 6);
     if (s >= a.G) return;
-    place_section(a, s, (int)(threadIdx.x & 63));
+    place_section<false>(a, s, (int)(threadIdx.x & 63));
 }
 
 __global__ __launch_bounds__(256) void seg_degenerate(SegArgs g, SegMap m) {

@@ -8,10 +8,10 @@
 
 namespace atg {
 
-// Everything a build kernel needs, passed by value (all device pointers).
 // Pack methods (the values of atg_pack_method).
 enum PackMethod : int32_t { PACK_CHUNKED = 0, PACK_BASIC = 1, PACK_NO_WEIGHTS = 2 };
 
+// Everything a build kernel needs, passed by value (all device pointers).
 struct BuildArgs {
     const void* weights;  // n weights (double or float)
     int dtype_f32;
@@ -42,7 +42,23 @@ struct BuildArgs {
     int32_t learned_segments;
     int32_t* knot_j;          // learned_segments + 1 exact split j at the knots
     int32_t pack_method;      // atg_pack_method
+    // Lean light lists: the placement writes only the light prefixes; the
+    // light items and weights stay in the staging, found through sec_loff
+    // (G + 1 values: the lights before section s, then the light count).
+    int32_t lean_lights;
+    uint32_t* sec_loff;
+    uint32_t* sec_ldir;       // light_dir_size(n): the section of light q << kLightDirShift
 };
+
+constexpr int kLightDirShift = 10;
+inline int64_t light_dir_size(int64_t n) { return (n >> kLightDirShift) + 2; }
+
+// The lean light lists need every pack chain to fit the staged pack's LDS
+// pool (chains hold at most n / p + 3 items, the pool at least
+// min(64 (n / p + 1), 1024)).
+inline bool lean_lights_ok(int64_t n, int64_t p_req, int32_t pack_method) {
+    return pack_method == PACK_CHUNKED && p_req > 0 && n / p_req <= 256;
+}
 
 // Section layout of PSA+ (AliasTableSplitGpu.cu:356-357, PsaPlusNormal.cu:27-29).
 inline void section_layout(int64_t n, int64_t param, int64_t* G, int64_t* Ls) {
@@ -89,6 +105,8 @@ hipError_t launch_split(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split_learned(const BuildArgs& a, hipStream_t s);
 hipError_t launch_split_pary(const BuildArgs& a, hipStream_t s);
 hipError_t launch_pack(const BuildArgs& a, hipStream_t s);
+// Lean light lists written out as lists (Li, Lw) from the staging.
+hipError_t launch_lean_lights_fill(const BuildArgs& a, hipStream_t s);
 // Chunked split and pack (split/pack interleave): blocks [b0, b1) of
 // split_blocks (256 splits each) or pack_blocks (64 chains each).
 int64_t split_blocks(const BuildArgs& a);

@@ -67,11 +67,12 @@ __global__ __launch_bounds__(256) void cert_reduce(BuildArgs a) {
 }
 
 // Four sections per block, one per wave.
+template <bool LEAN>
 __global__ __launch_bounds__(256) void partition_place(BuildArgs a) {
     if (a.st->error) return;
     const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (s >= a.G) return;
-    place_section(a, s, (int)(threadIdx.x & 63));
+    place_section<LEAN>(a, s, (int)(threadIdx.x & 63));
 }
 
 // ---------------------------------------------------------------------------
@@ -171,9 +172,19 @@ hipError_t launch_partition(const BuildArgs& a, hipStream_t s) {
 hipError_t launch_partition_place(const BuildArgs& a, hipStream_t s) {
     for (int b0 = 0; (a.G >> b0) > 1; b0 += 9)
         hipLaunchKernelGGL(block_tree, dim3(blocks_for(a.G >> b0, 512)), dim3(256), 0, s, a, b0);
-    hipLaunchKernelGGL(partition_place, dim3(blocks_for(a.G, 4)), dim3(256), 0, s, a);
+    if (a.lean_lights) hipLaunchKernelGGL(partition_place<true>, dim3(blocks_for(a.G, 4)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(partition_place<false>, dim3(blocks_for(a.G, 4)), dim3(256), 0, s, a);
     if (a.want_cert)
         hipLaunchKernelGGL(cert_reduce, dim3(blocks_for(a.G, 256 * 16)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void lean_lights_fill(BuildArgs a) {
+    lean_lights_fill_block(a, blockIdx.x, gridDim.x);
+}
+
+hipError_t launch_lean_lights_fill(const BuildArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(lean_lights_fill, dim3(1024), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -427,6 +427,9 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
 // completed dyadic blocks, then the staged leftovers with prefixes
 // P(s) + local (the canonical chunked scan recomputed in the same order),
 // and for the last section the totals, sentinels and split inputs.
+// LEAN: the lean light lists (a.lean_lights), fixed at compile time so that
+// each kernel carries one path.
+template <bool LEAN>
 __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int lane) {
     DevState* st = a.st;
     const int64_t start = min(a.n, s * a.Ls);
@@ -440,6 +443,7 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
     // and last prefix of each class; the pairs across sections are closed by
     // cert_reduce_block.
     double ext_l = 0.0, ext_h = __builtin_inf();
+#pragma unroll 1
     for (int hv = 0; hv < 2; ++hv) {
         const bool heavy = hv == 1;
         const int m = heavy ? mH : mL;
@@ -450,6 +454,8 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
         double* ow = heavy ? a.Hw : a.Lw;
         double* op = heavy ? a.Hp : a.Lp;
         const int64_t goff = heavy ? goffH : goffL;
+        // Lean lists: the lights keep their items and weights in the staging.
+        const bool full = heavy || !LEAN;
         double ext = heavy ? __builtin_inf() : 0.0;
         // Groups of PLACE_CHUNKS chunks of 64: all loads of a group in flight
         // together, the chunk scans independent, then the carries chained left
@@ -466,7 +472,7 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
                 item[u] = 0;
                 if (r < m) {
                     v[u] = __builtin_nontemporal_load(&a.stage_w[src + r]);
-                    item[u] = __builtin_nontemporal_load(&a.stage_i[src + r]);
+                    if (full) item[u] = __builtin_nontemporal_load(&a.stage_i[src + r]);
                 }
             }
 #pragma unroll
@@ -480,8 +486,10 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
                     const double pref = Pv + local;
                     const double carry_next = carry + readlane_d(inc[u], 63);
                     if (r < m) {
-                        oi[goff + r] = item[u];
-                        ow[goff + r] = v[u];
+                        if (full) {
+                            oi[goff + r] = item[u];
+                            ow[goff + r] = v[u];
+                        }
                         op[goff + r] = pref;
                     }
                     if (a.want_cert) {
@@ -514,11 +522,19 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
             a.sec_cert[6 * s + 4] = d2u(P.h);
         }
     }
+    if (LEAN) {
+        if (lane == 0) a.sec_loff[s] = (uint32_t)goffL;
+        // The directory entries of the lights 1024 q this section holds.
+        const int64_t q0 = (goffL + (1 << kLightDirShift) - 1) >> kLightDirShift;
+        const int64_t q1 = mL > 0 ? (goffL + mL - 1) >> kLightDirShift : q0 - 1;
+        for (int64_t q = q0 + lane; q <= q1; q += 64) a.sec_ldir[q] = (uint32_t)s;
+    }
     if (s == a.G - 1) {
         // Totals P(G), sentinels (AliasTableSplitGpu.cu:366-385) and the split inputs.
         const Prefix Tot = block_prefix_done(a, a.G, lane);
         if (lane == 0) {
             const int64_t nLr = (int64_t)(uint32_t)Tot.c, nHr = (int64_t)(Tot.c >> 32);
+            if (LEAN) a.sec_loff[a.G] = (uint32_t)nLr;
             a.Li[nLr] = (int32_t)(a.n + 1); a.Lw[nLr] = 0.0; a.Lp[nLr] = Tot.l;
             a.Hi[nHr] = (int32_t)a.n; a.Hw[nHr] = __builtin_inf(); a.Hp[nHr] = Tot.h;
             const int64_t numL = nLr + 1, numH = nHr + 1;
@@ -638,6 +654,79 @@ struct NoWeightsLists {
         return (uint32_t)it < (uint32_t)n ? (double)w[it] : __builtin_inf();
     }
 };
+// Lean light lists: light i lives in the staging of section s, the last
+// section with sec_loff[s] <= i (the non-empty one), at position
+// min(n, s Ls) + i - sec_loff[s].
+
+// The directory sec_ldir[q] holds the section of light q << kLightDirShift, a
+// section whose offset is at most any light of that block of lights.
+// Staging positions of lights g0 + 64 u + lane (u < U; those at or past m
+// from g0 are left alone), starting from a section s0 whose offset is at most
+// g0: one window of section offsets per 63 sections, each lane finding its
+// section by halving over the window.  Returns the section of light g0.
+template <int U>
+__device__ __forceinline__ int64_t light_positions(const BuildArgs& a, int64_t s0, uint32_t g0, int m, int lane,
+                                                   int32_t (&pos)[U]) {
+    int64_t first = -1;
+    bool done[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) done[u] = 64 * u + lane >= m;
+    for (int64_t s = s0;; s += 63) {
+        const uint32_t w = a.sec_loff[min(s + lane, a.G)];
+        bool pending = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t g = g0 + (uint32_t)(64 * u + lane);
+            int d = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1) {
+                const uint32_t v = (uint32_t)__shfl((int)w, min(d + step, 63), 64);
+                if (d + step <= 63 && v <= g) d += step;
+            }
+            const uint32_t wd = (uint32_t)__shfl((int)w, d, 64);
+            if (u == 0 && first < 0) {
+                const int d0 = __shfl(d, 0, 64);
+                if (d0 < 63) first = s + d0;
+            }
+            if (!done[u]) {
+                if (d < 63) {
+                    pos[u] = (int32_t)(min(a.n, (s + d) * a.Ls) + (int64_t)(g - wd));  // < n < 2^31
+                    done[u] = true;
+                } else {
+                    pending = true;
+                }
+            }
+        }
+        if (__ballot(pending) == 0ull) break;
+    }
+    return first;
+}
+
+// Per-lane access to lean lists (the pack's fallback for non-monotone
+// splits): a bisection over sec_loff per light.
+struct LeanLists {
+    const int32_t* Hi;
+    const double* Hw;
+    const uint32_t* loff;
+    const double* sw;
+    const int32_t* si;
+    int64_t G, Ls, n;
+    __device__ int64_t lpos(int i) const {
+        int64_t lo = 0, hi = G - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (loff[mid] <= (uint32_t)i) lo = mid;
+            else hi = mid - 1;
+        }
+        return min(n, lo * Ls) + (int64_t)((uint32_t)i - loff[lo]);
+    }
+    __device__ bool sentinel(int i) const { return (uint32_t)i >= loff[G]; }
+    __device__ int li(int i) const { return sentinel(i) ? (int)(n + 1) : si[lpos(i)]; }
+    __device__ double lw(int i) const { return sentinel(i) ? 0.0 : sw[lpos(i)]; }
+    __device__ int hi(int j) const { return Hi[j]; }
+    __device__ double hw(int j) const { return Hw[j]; }
+};
+
 struct StagedLists {
     const int32_t *Li, *Hi;
     const double *Lw, *Hw;
@@ -876,10 +965,42 @@ __device__ __forceinline__ void degenerate_block(const BuildArgs& a, int64_t blk
     const DevState* st = a.st;
     if (st->error || !st->degenerate) return;
     const int64_t nLr = st->numL - 1, nHr = st->numH - 1;
+    if (a.lean_lights) {
+        // The lights from the staging, one wave per section; then the heavies.
+        const int lane = threadIdx.x & 63;
+        const int64_t wpb = blockDim.x >> 6;
+        for (int64_t s = blk * wpb + (threadIdx.x >> 6); s < a.G; s += nblk * wpb) {
+            const int64_t base = min(a.n, s * a.Ls);
+            const int64_t m = (int64_t)a.sec_loff[s + 1] - (int64_t)a.sec_loff[s];
+            for (int64_t r = lane; r < m; r += 64) {
+                const int32_t item = a.stage_i[base + r];
+                store_row<AOS>(a, item, a.stage_w[base + r], item);
+            }
+        }
+        for (int64_t t = blk * blockDim.x + threadIdx.x; t < nHr; t += nblk * blockDim.x)
+            store_row<AOS>(a, a.Hi[t], a.Hw[t], a.Hi[t]);
+        return;
+    }
     for (int64_t t = blk * blockDim.x + threadIdx.x; t < nLr + nHr; t += nblk * blockDim.x) {
         const int32_t item = (t < nLr) ? a.Li[t] : a.Hi[t - nLr];
         const double w = (t < nLr) ? a.Lw[t] : a.Hw[t - nLr];
         store_row<AOS>(a, item, w, item);
+    }
+}
+
+// The light items and weights of the lean lists written out as lists (for a
+// copy-out of the lists), one wave per section.
+__device__ __forceinline__ void lean_lights_fill_block(const BuildArgs& a, int64_t blk, int64_t nblk) {
+    if (a.st->error) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    for (int64_t s = blk * wpb + (threadIdx.x >> 6); s < a.G; s += nblk * wpb) {
+        const int64_t base = min(a.n, s * a.Ls), l0 = a.sec_loff[s];
+        const int64_t m = (int64_t)a.sec_loff[s + 1] - l0;
+        for (int64_t r = lane; r < m; r += 64) {
+            a.Li[l0 + r] = a.stage_i[base + r];
+            a.Lw[l0 + r] = a.stage_w[base + r];
+        }
     }
 }
 
@@ -925,8 +1046,12 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
     }
     const double W_N = st->W_N;
     const GlobalLists gl{a.Li, a.Hi, a.Lw, a.Hw};
+    const LeanLists ll{a.Hi, a.Hw, a.sec_loff, a.stage_w, a.stage_i, a.G, a.Ls, a.n};
     if (__ballot(mine && !(prev.i <= cur.i && prev.j <= cur.j)) != 0ull) {
-        if (mine) pack_chain<AOS, false>(a, gl, prev, cur, W_N);
+        if (mine) {
+            if (a.lean_lights) pack_chain<AOS, false>(a, ll, prev, cur, W_N);
+            else pack_chain<AOS, false>(a, gl, prev, cur, W_N);
+        }
         return;
     }
     const int cap = a.pack_cap;
@@ -934,12 +1059,16 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
     int32_t* sI = reinterpret_cast<int32_t*>(sW + cap);
     int32_t* sA = sI + cap;
     int ta = 0;
+    int64_t lsec = -1;  // lean lists: the section of the next light to load
     while (ta < nch) {
         const int Ia = __shfl(prev.i, ta, 64), Ja = __shfl(prev.j, ta, 64);
         const bool fits = mine && lane >= ta && (cur.i - Ia) + (cur.j - Ja + 1) <= cap;
         const int tb = ta + __popcll(__ballot(fits));
         if (tb == ta) {
-            if (lane == ta) pack_chain<AOS, false>(a, gl, prev, cur, W_N);
+            if (lane == ta) {
+                if (a.lean_lights) pack_chain<AOS, false>(a, ll, prev, cur, W_N);
+                else pack_chain<AOS, false>(a, gl, prev, cur, W_N);
+            }
             ta += 1;
             continue;
         }
@@ -949,13 +1078,27 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
             const int m = hv ? nHu : nLu, base = hv ? nLu : 0;
             const double* gw = hv ? a.Hw + Ja : a.Lw + Ia;
             const int32_t* gi = hv ? a.Hi + Ja : a.Li + Ia;
+            const bool lean = hv == 0 && a.lean_lights;
             for (int t0 = 0; t0 < m; t0 += 8 * 64) {
                 double wv[8];
                 int32_t iv[8];
+                if (lean) {
+                    // The lights from the staging of their sections.
+                    const uint32_t g0 = (uint32_t)(Ia + t0);
+                    if (lsec < 0) lsec = a.sec_ldir[g0 >> kLightDirShift];
+                    int32_t pos[8];
+                    lsec = light_positions<8>(a, lsec, g0, m - t0, lane, pos);
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int t = t0 + u * 64 + lane;
-                    if (t < m) { wv[u] = gw[t]; iv[u] = gi[t]; }
+                    for (int u = 0; u < 8; ++u) {
+                        const int t = t0 + u * 64 + lane;
+                        if (t < m) { wv[u] = a.stage_w[pos[u]]; iv[u] = a.stage_i[pos[u]]; }
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int t = t0 + u * 64 + lane;
+                        if (t < m) { wv[u] = gw[t]; iv[u] = gi[t]; }
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {

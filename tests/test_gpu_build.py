@@ -193,7 +193,7 @@ def test_autotune_leaves_table_unchanged(gpu, oracle, monkeypatch, dist):
     ref = oracle.build(w, want_lists=False, **shape)
     for rep in range(2):
         knobs, ms = atg.autotune(w, reps=1, **shape)
-        assert ms > 0 and knobs["split"] in ("bisect", "learned")
+        assert ms > 0 and knobs["split"] in ("bisect", "learned", "pary")
         t = gpu_build(w, split=knobs["split"], learned_segments=knobs["learned_segments"],
                       interleave=knobs["interleave"], **shape)
         tw, ta = t.table()
