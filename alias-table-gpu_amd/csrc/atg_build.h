@@ -80,6 +80,9 @@ inline int32_t pack_cap_items(int64_t n, int64_t p_req) {
     int64_t cap = (64 * (per + 1) + 2 + 15) / 16 * 16;
     return (int32_t)(cap < kPackPoolItems ? cap : kPackPoolItems);
 }
+// A pack round that fits fewer chains than this into the pool hands the
+// wave's remaining chains to their lanes, each reading the lists from HBM.
+constexpr int kPackMinStagedChains = 8;
 // Weight (8 B), item (4 B) and recorded alias (4 B) per pool item.
 inline size_t pack_lds_bytes(int32_t cap) { return (size_t)cap * 16; }
 

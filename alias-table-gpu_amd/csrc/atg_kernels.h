@@ -1064,6 +1064,13 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
         const int Ia = __shfl(prev.i, ta, 64), Ja = __shfl(prev.j, ta, 64);
         const bool fits = mine && lane >= ta && (cur.i - Ia) + (cur.j - Ja + 1) <= cap;
         const int tb = ta + __popcll(__ballot(fits));
+        if (tb - ta < kPackMinStagedChains && !a.lean_lights) {
+            // Long chains: so few fit the pool at once that the rounds would
+            // run the wave's chains nearly one after another; every lane
+            // runs its own chain from the lists in HBM instead, all at once.
+            if (mine && lane >= ta) pack_chain<AOS, false>(a, gl, prev, cur, W_N);
+            break;
+        }
         if (tb == ta) {
             if (lane == ta) {
                 if (a.lean_lights) pack_chain<AOS, false>(a, ll, prev, cur, W_N);
