@@ -158,15 +158,18 @@ extern "C" atg_status atg_autotune(const void* weights, atg_mem mem, int64_t n, 
     if (rc != ATG_OK) return rc;
     Events ev;
     if ((rc = ev.init()) != ATG_OK) return rc;
-    // Candidates: bisection, the learned index with 1024 and 4096 knots and
-    // the P-ary search, each with and without the split/pack interleave.
+    // Candidates: bisection, the learned index with 1024 to 65536 knots (at
+    // most one per 16 splits are used) and the P-ary search, each with and
+    // without the split/pack interleave.
     struct Cand { int32_t split, knots, inter; };
-    const Cand cands[] = {{ATG_SPLIT_BISECT, 4096, 0}, {ATG_SPLIT_BISECT, 4096, 1},
-                          {ATG_SPLIT_LEARNED, 1024, 0}, {ATG_SPLIT_LEARNED, 4096, 0},
-                          {ATG_SPLIT_LEARNED, 1024, 1}, {ATG_SPLIT_LEARNED, 4096, 1},
-                          {ATG_SPLIT_PARY, 4096, 0}, {ATG_SPLIT_PARY, 4096, 1}};
+    const Cand cands[] = {{ATG_SPLIT_BISECT, 4096, 0},     {ATG_SPLIT_BISECT, 4096, 1},
+                          {ATG_SPLIT_LEARNED, 1024, 0},    {ATG_SPLIT_LEARNED, 4096, 0},
+                          {ATG_SPLIT_LEARNED, 16384, 0},   {ATG_SPLIT_LEARNED, 65536, 0},
+                          {ATG_SPLIT_LEARNED, 1024, 1},    {ATG_SPLIT_LEARNED, 4096, 1},
+                          {ATG_SPLIT_LEARNED, 16384, 1},   {ATG_SPLIT_LEARNED, 65536, 1},
+                          {ATG_SPLIT_PARY, 4096, 0},       {ATG_SPLIT_PARY, 4096, 1}};
     atg_build_opts co = o;
-    co.learned_segments = 4096;
+    co.learned_segments = 65536;
     co.interleave = 0;
     atg_table* t = nullptr;
     if ((rc = atg_create(&t, n, dtype, layout, &co)) != ATG_OK) return rc;

@@ -86,7 +86,7 @@ def headline(args, stream, w):
     import atg
     n = args.n
     shape = dict(section_size=args.section_size, chains=args.chains, num_splits=n // args.items_per_split)
-    knobs = {"split": args.split, "learned_segments": 4096, "interleave": False}
+    knobs = {"split": args.split, "learned_segments": args.learned_segments, "interleave": False}
     if not args.no_autotune:
         t0 = time.perf_counter()
         knobs, tms = atg.autotune(w, reps=3, **shape)
@@ -223,32 +223,36 @@ def bench_learned(args, stream):
     res = {"workload": f"N={n:.0e} heavy_mix (1% heavy, 90% mass), {args.learned_segments} segments",
            "section_size": shape["section_size"], "chains": shape["chains"], "items_per_split": args.items_per_split}
     tabs = {}
-    for split in ("bisect", "learned", "pary"):
-        t = atg.Table(n, split=split, learned_segments=args.learned_segments, record_phases=True, **shape)
+    for name, split, knots in (("bisect", "bisect", args.learned_segments),
+                               ("learned", "learned", args.learned_segments),
+                               ("learned_65536", "learned", 65536), ("pary", "pary", args.learned_segments)):
+        t = atg.Table(n, split=split, learned_segments=knots, record_phases=True, **shape)
         t.set_weights(w, stream)
         for _ in range(args.warmup):
             t.build(stream)
         torch.cuda.synchronize()
         ms, mean = time_events(stream, lambda: t.build(stream), args.steps)
         st = t.stats()
-        res[split] = {"build_ms": round(ms, 4), "split_ms": round(st["phase_ms"][3], 4),
-                      "gitems_s": round(n / (ms * 1e-3) / 1e9, 3), "p": st["p"]}
+        res[name] = {"build_ms": round(ms, 4), "split_ms": round(st["phase_ms"][3], 4),
+                     "gitems_s": round(n / (ms * 1e-3) / 1e9, 3), "p": st["p"]}
         if split != "bisect":
-            res[split]["search_fallbacks"] = st["learned_fallbacks"]
-            res[split]["monotone"] = st["monotone"]
-        if split == "learned":
+            res[name]["search_fallbacks"] = st["learned_fallbacks"]
+            res[name]["monotone"] = st["monotone"]
+        if name == "learned":
             rel, ok = t.post_build(verify=True)
             res["max_rel_err"] = rel
             res["reference_check_ok"] = ok
-        tabs[split] = t.table()
+        tabs[name] = t.table()
         del t
         torch.cuda.empty_cache()
     res["speedup"] = round(res["bisect"]["build_ms"] / res["learned"]["build_ms"], 4)
     res["split_speedup"] = round(res["bisect"]["split_ms"] / max(res["learned"]["split_ms"], 1e-9), 3)
+    res["speedup_65536_knots"] = round(res["bisect"]["build_ms"] / res["learned_65536"]["build_ms"], 4)
     res["pary_speedup"] = round(res["bisect"]["build_ms"] / res["pary"]["build_ms"], 4)
     a = tabs["bisect"]
     res["bit_identical"] = all(bool(np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64)) and
-                                    np.array_equal(a[1], b[1])) for b in (tabs["learned"], tabs["pary"]))
+                                    np.array_equal(a[1], b[1]))
+                               for b in (tabs["learned"], tabs["learned_65536"], tabs["pary"]))
     log(f"[bench] learned vs plain: {res}")
     if not args.no_rejection_fills:
         res["rejection_fills"] = bench_rejection_fills(args, stream, w, f"N={n:.0e} heavy_mix (1% heavy, 90% mass)")

@@ -42,16 +42,19 @@ def test_headline_build_matches_oracle(gpu, oracle, pareto):
 
 def test_learned_split_identical_at_full_size(gpu):
     w = atg.generate_weights("heavy_mix", N, seed=42, param=0.9)
-    shape = dict(chains=16, section_size=2048, num_splits=N // 64, learned_segments=4096)
+    shape = dict(chains=16, section_size=2048, num_splits=N // 64)
     tabs = []
-    for split in ("bisect", "learned"):
-        t = atg.Table(N, split=split, **shape)
+    # The learned split with 4096 knots and with 65536 (the most its rule of
+    # one knot per 16 splits allows here is 97,656), and the P-ary split.
+    for split, knots in (("bisect", 4096), ("learned", 4096), ("learned", 65536), ("pary", 4096)):
+        t = atg.Table(N, split=split, learned_segments=knots, **shape)
         t.set_weights(w)
         t.build()
         t.sync()
         st = t.stats()
-        if split == "learned":
-            assert st["monotone"] == 1 and st["learned_fallbacks"] == 0
+        if split != "bisect":
+            assert st["monotone"] == 1 and st["learned_fallbacks"] == 0, (split, knots)
         tabs.append(t.table())
         t.close()
-    assert np.array_equal(bits(tabs[0][0]), bits(tabs[1][0])) and np.array_equal(tabs[0][1], tabs[1][1])
+    for k in range(1, len(tabs)):
+        assert np.array_equal(bits(tabs[0][0]), bits(tabs[k][0])) and np.array_equal(tabs[0][1], tabs[k][1]), k
