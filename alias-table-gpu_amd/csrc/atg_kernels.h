@@ -662,16 +662,37 @@ struct NoWeightsLists {
 // section whose offset is at most any light of that block of lights.
 // Staging positions of lights g0 + 64 u + lane (u < U; those at or past m
 // from g0 are left alone), starting from a section s0 whose offset is at most
-// g0: one window of section offsets per 63 sections, each lane finding its
-// section by halving over the window.  Returns the section of light g0.
+// g0.  A group that spans a few sections walks them (their bounds uniform,
+// so scalar loads), each lane taking the position of its light from the
+// section that holds it; past kLightWalk sections the rest is found from
+// windows of 63 section offsets, each lane halving over the window.  Returns
+// a section whose offset is at most the group's last light, the starting
+// point for the next group.
+constexpr int kLightWalk = 8;
 template <int U>
 __device__ __forceinline__ int64_t light_positions(const BuildArgs& a, int64_t s0, uint32_t g0, int m, int lane,
                                                    int32_t (&pos)[U]) {
-    int64_t first = -1;
+    const uint32_t gEnd = g0 + (uint32_t)min(m, 64 * U);  // the group's lights are [g0, gEnd)
+    int64_t s = __builtin_amdgcn_readfirstlane((int)s0);
+    uint32_t lo = a.sec_loff[s], hi = a.sec_loff[s + 1];
+    for (int walk = 0;; ++walk) {
+        const int32_t base = (int32_t)min(a.n, s * a.Ls) - (int32_t)lo;  // position of light g: base + g
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t g = g0 + (uint32_t)(64 * u + lane);
+            if (g >= lo && g < hi) pos[u] = base + (int32_t)g;
+        }
+        if (hi >= gEnd) return s;
+        ++s;
+        lo = hi;
+        if (walk + 1 == kLightWalk) break;
+        hi = a.sec_loff[s + 1];
+    }
+    // Lights [lo, gEnd) are left; sec_loff[s] = lo.
     bool done[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) done[u] = 64 * u + lane >= m;
-    for (int64_t s = s0;; s += 63) {
+    for (int u = 0; u < U; ++u) done[u] = 64 * u + lane >= m || g0 + (uint32_t)(64 * u + lane) < lo;
+    for (;; s += 63) {
         const uint32_t w = a.sec_loff[min(s + lane, a.G)];
         bool pending = false;
 #pragma unroll
@@ -684,10 +705,6 @@ __device__ __forceinline__ int64_t light_positions(const BuildArgs& a, int64_t s
                 if (d + step <= 63 && v <= g) d += step;
             }
             const uint32_t wd = (uint32_t)__shfl((int)w, d, 64);
-            if (u == 0 && first < 0) {
-                const int d0 = __shfl(d, 0, 64);
-                if (d0 < 63) first = s + d0;
-            }
             if (!done[u]) {
                 if (d < 63) {
                     pos[u] = (int32_t)(min(a.n, (s + d) * a.Ls) + (int64_t)(g - wd));  // < n < 2^31
@@ -697,9 +714,8 @@ __device__ __forceinline__ int64_t light_positions(const BuildArgs& a, int64_t s
                 }
             }
         }
-        if (__ballot(pending) == 0ull) break;
+        if (__ballot(pending) == 0ull) return s;
     }
-    return first;
 }
 
 // Per-lane access to lean lists (the pack's fallback for non-monotone
