@@ -83,8 +83,9 @@ inline int32_t pack_cap_items(int64_t n, int64_t p_req) {
 // A pack round that fits fewer chains than this into the pool hands the
 // wave's remaining chains to their lanes, each reading the lists from HBM.
 constexpr int kPackMinStagedChains = 8;
-// Weight (8 B), item (4 B) and recorded alias (4 B) per pool item.
-inline size_t pack_lds_bytes(int32_t cap) { return (size_t)cap * 16; }
+// Weight (8 B), item (4 B) and recorded alias (4 B) per pool item, and the
+// spare weight slot the light steps record into (16 B keep the alignment).
+inline size_t pack_lds_bytes(int32_t cap) { return (size_t)cap * 16 + 16; }
 
 // Knots of the learned split index: at most `opt`, and at most one per 16
 // requested splits (at least 64).  Each knot is a full bisection, so a table

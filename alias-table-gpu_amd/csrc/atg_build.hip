@@ -38,12 +38,14 @@ __global__ __launch_bounds__(1024) void sum_finish(BuildArgs a) {
 //    the blocks of s's binary digits from high to low (pass 2).
 // As many waves as fit the chip at once; wave w runs sections w, w + grid,
 // ... and loads each next section while the current one's chains run.
-template <typename T, bool AOS>
+// XU: weights per lane held for a section (sections of up to 64 XU items);
+// the short-section kernel keeps fewer registers, so more waves fit a SIMD.
+template <typename T, bool AOS, int XU>
 __global__ __launch_bounds__(64) void partition_local(BuildArgs a) {
     extern __shared__ double lds_raw[];
     if (a.st->error) return;
     const int lane = (int)threadIdx.x;
-    T x[PSA_MAX_LEN / 64];
+    T x[XU];
     int64_t s = blockIdx.x;
     if (s < a.G) section_load<T>(a, s, lane, x);
     for (; s < a.G; s += gridDim.x) {
@@ -111,10 +113,10 @@ __global__ __launch_bounds__(64) void pack_basic(BuildArgs a, int64_t blk0) {
     const Split prev = a.splits[k], cur = a.splits[k + 1];
     if (NOW) {
         const NoWeightsLists<T> l{a.Li, a.Hi, (const T*)a.weights, a.n};
-        pack_chain<AOS, false>(a, l, prev, cur, st->W_N);
+        pack_chain<AOS>(a, l, prev, cur, st->W_N);
     } else {
         const GlobalLists l{a.Li, a.Hi, a.Lw, a.Hw};
-        pack_chain<AOS, false>(a, l, prev, cur, st->W_N);
+        pack_chain<AOS>(a, l, prev, cur, st->W_N);
     }
 }
 
@@ -154,10 +156,18 @@ int64_t block_tree_total(int64_t G) {
 // The partition phase in its two passes: the local pass (partition, PSA+
 // chains, staged leftovers, level-0 block sums), then the block tree and the
 // placement of the leftover lists (and the split certificate).
+static void (*partition_local_kernel(bool f32, bool aos, int64_t Ls_max))(BuildArgs) {
+    constexpr int S = kShortSectionMax / 64, L = PSA_MAX_LEN / 64;
+    if (Ls_max <= kShortSectionMax)
+        return f32 ? (aos ? partition_local<float, true, S> : partition_local<float, false, S>)
+                   : (aos ? partition_local<double, true, S> : partition_local<double, false, S>);
+    return f32 ? (aos ? partition_local<float, true, L> : partition_local<float, false, L>)
+               : (aos ? partition_local<double, true, L> : partition_local<double, false, L>);
+}
+
 hipError_t launch_partition_local(const BuildArgs& a, hipStream_t s) {
     const size_t lds = psa_lds_bytes(a.Ls_max);
-    void (*k)(BuildArgs) = a.dtype_f32 ? (a.rows ? partition_local<float, true> : partition_local<float, false>)
-                                       : (a.rows ? partition_local<double, true> : partition_local<double, false>);
+    void (*k)(BuildArgs) = partition_local_kernel(a.dtype_f32 != 0, a.rows != nullptr, a.Ls_max);
     const dim3 g((unsigned)std::min<int64_t>(a.G, resident_blocks((const void*)k, 64, lds)));
     hipLaunchKernelGGL(k, g, dim3(64), lds, s, a);
     return hipGetLastError();
@@ -266,8 +276,11 @@ hipError_t launch_join_rows(const double* w, const int32_t* al, Row* rows, int64
 }
 
 hipError_t set_partition_lds_limit(size_t bytes) {
-    const void* fns[] = {(const void*)partition_local<double, true>, (const void*)partition_local<double, false>,
-                         (const void*)partition_local<float, true>, (const void*)partition_local<float, false>};
+    constexpr int S = kShortSectionMax / 64, L = PSA_MAX_LEN / 64;
+    const void* fns[] = {(const void*)partition_local<double, true, L>, (const void*)partition_local<double, false, L>,
+                         (const void*)partition_local<float, true, L>, (const void*)partition_local<float, false, L>,
+                         (const void*)partition_local<double, true, S>, (const void*)partition_local<double, false, S>,
+                         (const void*)partition_local<float, true, S>, (const void*)partition_local<float, false, S>};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return e;

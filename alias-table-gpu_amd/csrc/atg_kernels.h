@@ -54,6 +54,9 @@ __device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t
 
 
 #define PSA_MAX_LEN 2048
+// Sections of at most this many items run the partition kernel that holds
+// half the weight registers.
+constexpr int64_t kShortSectionMax = 1024;
 // Chunks of 64 leftovers a wave keeps in flight at once when staging or
 // placing them.
 #define PLACE_GROUP 8
@@ -209,13 +212,13 @@ __device__ __forceinline__ void chain_positions(int r0, int off, int lo, int* po
 // The weights of section s, lane-strided (item 64 u + lane in x[u]), in
 // their own type (converted to FP64 where they are used, so that no
 // conversion waits for its load); all loads in flight at once.
-template <typename T>
-__device__ __forceinline__ void section_load(const BuildArgs& a, int64_t s, int lane, T (&x)[PSA_MAX_LEN / 64]) {
+template <typename T, int XU>
+__device__ __forceinline__ void section_load(const BuildArgs& a, int64_t s, int lane, T (&x)[XU]) {
     const T* w = (const T*)a.weights;
     const int64_t start = min(a.n, s * a.Ls);
     const int len = (int)(min(a.n, (s + 1) * a.Ls) - start);
 #pragma unroll
-    for (int u = 0; u < PSA_MAX_LEN / 64; ++u) {
+    for (int u = 0; u < XU; ++u) {
         const int k = (u << 6) + lane;
         x[u] = (k < len) ? w[start + k] : T(0);
     }
@@ -228,8 +231,8 @@ __device__ __forceinline__ void section_load(const BuildArgs& a, int64_t s, int 
 // aggregates (stored as the level-0 block of the dyadic section tree) and the
 // leftovers staged in rank order at the section's own range (lights, then
 // heavies).  Nothing here waits for another section's work.
-template <typename T, bool AOS, typename AfterPartition>
-__device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x)[PSA_MAX_LEN / 64],
+template <typename T, bool AOS, int XU, typename AfterPartition>
+__device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x)[XU],
                                             double* lds_raw, int lane, AfterPartition after_partition) {
     DevState* st = a.st;
     const int64_t start = min(a.n, s * a.Ls);
@@ -249,7 +252,7 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
     __builtin_amdgcn_s_waitcnt(0x0F70);
     int nL = 0, nH = 0;
 #pragma unroll
-    for (int u = 0; u < PSA_MAX_LEN / 64; ++u) {
+    for (int u = 0; u < XU; ++u) {
         const int k = (u << 6) + lane;
         if ((u << 6) < len) {
             const bool valid = k < len;
@@ -284,15 +287,12 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
             jLo = t * stepsH; jHi = (t == C - 1) ? nH : (t + 1) * stepsH;
         }
         if (t < C && iHi - iLo > 0 && jHi - jLo > 0) {
-            // Registers hold the chain's current heavy (residual wc, item
-            // hj), the next heavy (hw1, hi1) and the current light (lw, li).
-            // Every step issues the loads of the light after the current one
-            // (nlw, nli) and of the heavy after the next one (nhw, nhi); the
-            // next step consumes one of the two, so LDS latency stays off the
-            // chain's critical path.  Indices are clamped to the chain's
-            // ranges; a clamped value is never consumed (the loop leaves
-            // first).  Lights and heavies share one LDS array (heavy j at
-            // len-1-j).
+            // Registers hold the chain's current heavy (residual wc, item hj),
+            // the next heavy (hw1, hi1) and the current light (lw, li); a step
+            // stores one row and reads the light and the next heavy at its
+            // new indices (within the chain's ranges) for the following step.
+            // The loop is one straight run of instructions besides its exit.
+            // Lights and heavies share one LDS array (heavy j at len-1-j).
             int i = iLo, j = jLo;
             double wc = L.Hw(j);
             int hj = L.Hi(j);
@@ -301,60 +301,22 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
             int q = min(j + 1, jHi - 1);
             double hw1 = L.Hw(q);
             int hi1 = L.Hi(q);
-            q = min(i + 1, iHi - 1);
-            double nlw = L.Lw(q);
-            int nli = L.Li(q);
-            q = min(j + 2, jHi - 1);
-            double nhw = L.Hw(q);
-            int nhi = L.Hi(q);
             const double thr = W_N + ATG_EPSILON;
             while (true) {
+                // Full bucket: the heavy's row, then the next heavy; else the
+                // light's row.  A step needs a further element on its side.
                 const bool ex = wc <= thr;
-                const int room = ex ? jHi - 1 - j : iHi - 1 - i;
-                if (room <= 0) break;
-                // While every chain still running takes light steps (its heavy
-                // is not exhausted and a light follows), a short loop does
-                // exactly those steps: the row of the light, the residual
-                // update, the next light.  Chains whose lights outnumber the
-                // heavies' excess spend most of their steps here.
-                const uint64_t running = __ballot(true);
-                if (__ballot(!ex) == running) {
-                    // Lights i + 1 (nlw, nli) and i + 2 (pw, pi) are in flight.
-                    int q2 = min(i + 2, iHi - 1);
-                    double pw = L.w[q2];
-                    int pi = L.idx[q2];
-                    do {
-                        store_row<AOS>(a, start + li, lw, (int32_t)(start + hj));
-                        wc += lw - W_N;
-                        i += 1;
-                        lw = nlw;
-                        li = nli;
-                        nlw = pw;
-                        nli = pi;
-                        q2 = min(i + 2, iHi - 1);
-                        pw = L.w[q2];
-                        pi = L.idx[q2];
-                    } while (__ballot(!(wc <= thr) && i < iHi - 1) == running);
-                    continue;
-                }
-                const int item = ex ? hj : li;
-                const double rw = ex ? wc : lw;
-                const int alias = ex ? hi1 : hj;
-                const double next = ex ? hw1 : lw;
-                wc += next - W_N;
-                store_row<AOS>(a, start + item, rw, (int32_t)(start + alias));
+                if ((ex ? jHi - 1 - j : iHi - 1 - i) <= 0) break;
+                store_row<AOS>(a, start + (ex ? hj : li), ex ? wc : lw, (int32_t)(start + (ex ? hi1 : hj)));
+                wc += (ex ? hw1 : lw) - W_N;
+                hj = ex ? hi1 : hj;
                 j += ex ? 1 : 0;
                 i += ex ? 0 : 1;
-                hj = ex ? hi1 : hj;
-                hw1 = ex ? nhw : hw1;
-                hi1 = ex ? nhi : hi1;
-                lw = ex ? lw : nlw;
-                li = ex ? li : nli;
-                const int qi = min(i + 1, iHi - 1), qj = len - 1 - min(j + 2, jHi - 1);
-                nlw = L.w[qi];
-                nli = L.idx[qi];
-                nhw = L.w[qj];
-                nhi = L.idx[qj];
+                lw = L.Lw(i);
+                li = L.Li(i);
+                q = min(j + 1, jHi - 1);
+                hw1 = L.Hw(q);
+                hi1 = L.Hi(q);
             }
             // Remainder (PsaPlusNormal.cu:112-123): back to L when its
             // residual is light and a consumed light slot can hold it,
@@ -743,16 +705,6 @@ struct LeanLists {
     __device__ double hw(int j) const { return Hw[j]; }
 };
 
-struct StagedLists {
-    const int32_t *Li, *Hi;
-    const double *Lw, *Hw;
-    int iA, jA;
-    __device__ int li(int i) const { return Li[i - iA]; }
-    __device__ double lw(int i) const { return Lw[i - iA]; }
-    __device__ int hi(int j) const { return Hi[j - jA]; }
-    __device__ double hw(int j) const { return Hw[j - jA]; }
-};
-
 // One pack chain (PackMethodBasic.cuh:59-126).  Registers hold the current
 // bucket weight w, the current heavy item hj, the next heavy (hw1, hi1) and
 // the current light (lw, li); every step issues the loads of the light after
@@ -761,11 +713,8 @@ struct StagedLists {
 // path.  Read indices are clamped to the chain's own ranges (lights up to
 // max(iUpper - 1, prev.i), heavies up to jUpper); a clamped value is never
 // consumed.
-// With REC, a light's row is not stored: its alias goes to rec[i - rbase]
-// and the caller writes the light rows afterwards in list order.
-template <bool AOS, bool REC, typename Lists>
-__device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, Split prev, Split cur, double W_N,
-                                           int32_t* rec = nullptr, int rbase = 0) {
+template <bool AOS, typename Lists>
+__device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, Split prev, Split cur, double W_N) {
     const int iUpper = cur.i, jUpper = cur.j;
     int i = prev.i, j = prev.j;
     const int iLast = max(iUpper - 1, i);
@@ -791,8 +740,7 @@ __device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, S
         const int alias = ex ? hi1 : hj;
         const double next = ex ? hw1 : lw;
         w += next - W_N;
-        if (REC && !ex) rec[i - rbase] = alias;
-        else if ((uint32_t)item < (uint32_t)a.n) store_row<AOS>(a, item, rw, alias);
+        if ((uint32_t)item < (uint32_t)a.n) store_row<AOS>(a, item, rw, alias);
         j += ex ? 1 : 0;
         i += ex ? 0 : 1;
         hj = ex ? hi1 : hj;
@@ -805,6 +753,49 @@ __device__ __forceinline__ void pack_chain(const BuildArgs& a, const Lists& l, S
         nli = l.li(qi);
         nhw = l.hw(qj);
         nhi = l.hi(qj);
+    }
+}
+
+// One pack chain over its round's LDS pool (the steps of pack_chain, with
+// pool indices: light i at i - Ia, heavy j at nLu + j - Ja).  Rows are not
+// stored here but recorded in the pool: each step writes its row's alias to
+// sA[slot] and, for a heavy, the residual weight to sW[slot] (a light's weight
+// is already there), so that the caller writes all rows in pool order.  A
+// light step writes its weight to the spare slot sW[spare] instead.  The
+// chain's last heavy (jUpper) is the next chain's first, whose slot that
+// chain overwrites: its weight is read once, before any chain steps, and
+// taken from a register.  The loop's only branch is its exit.
+__device__ __forceinline__ void pack_chain_pool(double* sW, const int32_t* sI, int32_t* sA, int spare, int nLu,
+                                                int Ia, int Ja, Split prev, Split cur, double W_N) {
+    const int iUpper = cur.i - Ia, jUpper = nLu + cur.j - Ja;
+    int i = prev.i - Ia, j = nLu + prev.j - Ja;
+    const int iLast = max(iUpper - 1, i);  // light reads are clamped to the chain's range
+    const double hwLast = sW[jUpper];
+    double w = prev.spill;
+    if (prev.spill == 0) w = sW[j];
+    int hj = sI[j];
+    double lw = sW[min(i, iLast)];
+    int q = min(j + 1, jUpper);
+    double hw1 = sW[q];  // the next heavy's weight, unless it is the last (q == jUpper)
+    int hi1 = sI[q];
+    wave_lds_sync();  // every chain holds its last heavy's weight before any slot is overwritten
+    const double thr = W_N + ATG_EPSILON;
+    while (true) {
+        const bool idone = i >= iUpper, jdone = j >= jUpper;
+        if (idone && jdone) break;
+        // A heavy step when the bucket is full (or no light is left) and a
+        // heavy remains; a light step otherwise (pack_chain's rule).
+        const bool ex = idone || (w <= thr && !jdone);
+        sA[ex ? j : i] = ex ? hi1 : hj;
+        sW[ex ? j : spare] = w;
+        w += (ex ? (q == jUpper ? hwLast : hw1) : lw) - W_N;
+        hj = ex ? hi1 : hj;
+        j += ex ? 1 : 0;
+        i += ex ? 0 : 1;
+        lw = sW[min(i, iLast)];
+        q = min(j + 1, jUpper);
+        hw1 = sW[q];
+        hi1 = sI[q];
     }
 }
 
@@ -1065,14 +1056,15 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
     const LeanLists ll{a.Hi, a.Hw, a.sec_loff, a.stage_w, a.stage_i, a.G, a.Ls, a.n};
     if (__ballot(mine && !(prev.i <= cur.i && prev.j <= cur.j)) != 0ull) {
         if (mine) {
-            if (a.lean_lights) pack_chain<AOS, false>(a, ll, prev, cur, W_N);
-            else pack_chain<AOS, false>(a, gl, prev, cur, W_N);
+            if (a.lean_lights) pack_chain<AOS>(a, ll, prev, cur, W_N);
+            else pack_chain<AOS>(a, gl, prev, cur, W_N);
         }
         return;
     }
+    // The pool: weights (and a spare slot), items, aliases.
     const int cap = a.pack_cap;
     double* sW = lds_raw;
-    int32_t* sI = reinterpret_cast<int32_t*>(sW + cap);
+    int32_t* sI = reinterpret_cast<int32_t*>(sW + cap + 1);
     int32_t* sA = sI + cap;
     int ta = 0;
     int64_t lsec = -1;  // lean lists: the section of the next light to load
@@ -1084,13 +1076,13 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
             // Long chains: so few fit the pool at once that the rounds would
             // run the wave's chains nearly one after another; every lane
             // runs its own chain from the lists in HBM instead, all at once.
-            if (mine && lane >= ta) pack_chain<AOS, false>(a, gl, prev, cur, W_N);
+            if (mine && lane >= ta) pack_chain<AOS>(a, gl, prev, cur, W_N);
             break;
         }
         if (tb == ta) {
             if (lane == ta) {
-                if (a.lean_lights) pack_chain<AOS, false>(a, ll, prev, cur, W_N);
-                else pack_chain<AOS, false>(a, gl, prev, cur, W_N);
+                if (a.lean_lights) pack_chain<AOS>(a, ll, prev, cur, W_N);
+                else pack_chain<AOS>(a, gl, prev, cur, W_N);
             }
             ta += 1;
             continue;
@@ -1131,14 +1123,25 @@ __device__ __forceinline__ void pack_staged_block(const BuildArgs& a, int64_t bl
             }
         }
         wave_lds_sync();
-        if (lane >= ta && lane < tb) {
-            const StagedLists l{sI, sI + nLu, sW, sW + nLu, Ia, Ja};
-            pack_chain<AOS, true>(a, l, prev, cur, W_N, sA, Ia);
-        }
+        if (lane >= ta && lane < tb) pack_chain_pool(sW, sI, sA, cap, nLu, Ia, Ja, prev, cur, W_N);
         wave_lds_sync();
-        for (int r = lane; r < nLu; r += 64) {
-            const int it = sI[r];
-            if ((uint32_t)it < (uint32_t)a.n) store_row<AOS>(a, it, sW[r], sA[r]);
+        // The rows of the round in pool order (lights, then heavies but the
+        // last, which the next round's first chain starts from), four per
+        // lane at once; items rise along each list.
+        const int nRows = nLu + nHu - 1;
+        for (int r0 = 0; r0 < nRows; r0 += 4 * 64) {
+            int it[4], al[4];
+            double wv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = min(r0 + 64 * u + lane, nRows - 1);
+                it[u] = sI[r];
+                wv[u] = sW[r];
+                al[u] = sA[r];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (r0 + 64 * u + lane < nRows && (uint32_t)it[u] < (uint32_t)a.n) store_row<AOS>(a, it[u], wv[u], al[u]);
         }
         wave_lds_sync();
         ta = tb;
