@@ -83,6 +83,13 @@ inline int32_t pack_cap_items(int64_t n, int64_t p_req) {
 // A pack round that fits fewer chains than this into the pool hands the
 // wave's remaining chains to their lanes, each reading the lists from HBM.
 constexpr int kPackMinStagedChains = 8;
+// Chains of more than this many items (n / p) run the windowed pack: so few
+// of them fit a pool round that the staged pack would hand nearly all of them
+// to their lanes.
+constexpr int64_t kPackWindowedItems = 256;
+inline bool pack_windowed_ok(int64_t n, int64_t p_req, int32_t pack_method) {
+    return pack_method == PACK_CHUNKED && p_req > 0 && n / p_req > kPackWindowedItems;
+}
 // Weight (8 B), item (4 B) and recorded alias (4 B) per pool item, and the
 // spare weight slot the light steps record into (16 B keep the alignment).
 inline size_t pack_lds_bytes(int32_t cap) { return (size_t)cap * 16 + 16; }

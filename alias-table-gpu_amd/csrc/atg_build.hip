@@ -120,6 +120,136 @@ __global__ __launch_bounds__(64) void pack_basic(BuildArgs a, int64_t blk0) {
     }
 }
 
+// 3e. Pack of long chains (far more items per chain than a round of the
+//     LDS pool holds): every lane runs its own chain (pack_chain's steps)
+//     through windows of its lists staged in LDS.  A round reads the next
+//     kWinL lights and the current and next kWinH - 1 heavies of each lane's
+//     chain with all loads in flight at once; the lanes then step through
+//     their windows until a chain needs an item past its window, recording
+//     each row in the window (a light's alias; a heavy's alias and residual
+//     weight), and finally the wave writes the round's rows.  Windows are
+//     loaded and rows written a few chains per instruction, consecutive list
+//     entries on consecutive lanes, so that an instruction touches few lines.
+//     Blocks of 64 chains, as pack_staged.
+constexpr int kWinL = 16, kWinH = 8;
+template <bool AOS>
+__global__ __launch_bounds__(64) void pack_windowed(BuildArgs a, int64_t blk0) {
+    __shared__ double wl[kWinL][64], wh[kWinH][64];
+    __shared__ int32_t il[kWinL][64], ih[kWinH][64], al[kWinL][64], ah[kWinH][64];
+    const DevState* st = a.st;
+    if (st->error || st->degenerate) return;
+    const int64_t p = st->p;
+    const int lane = (int)threadIdx.x;
+    const int64_t k0 = (blk0 + blockIdx.x) * 64;
+    if (k0 + 1 > p) return;
+    const bool mine = k0 + lane < p;
+    Split prev{0, 0, 0.0}, cur{0, 0, 0.0};
+    if (mine) {
+        prev = a.splits[k0 + lane];
+        cur = a.splits[k0 + lane + 1];
+    }
+    const double W_N = st->W_N;
+    const GlobalLists gl{a.Li, a.Hi, a.Lw, a.Hw};
+    if (__ballot(mine && !(prev.i <= cur.i && prev.j <= cur.j)) != 0ull) {
+        if (mine) pack_chain<AOS>(a, gl, prev, cur, W_N);  // a bisection fell back: no windows
+        return;
+    }
+    const int iUpper = cur.i, jUpper = cur.j;
+    int i = prev.i, j = prev.j;
+    const int iLast = max(iUpper - 1, i);  // list reads are clamped as in pack_chain
+    double w = prev.spill;
+    if (mine && prev.spill == 0) w = a.Hw[j];
+    bool run = mine;
+    const double thr = W_N + ATG_EPSILON;
+    while (__ballot(run) != 0ull) {
+        // The windows: lights i .. i + kWinL - 1, heavies j .. j + kWinH - 1
+        // (slot 0: the current heavy, whose residual is w).
+        // Staged a window at a time: one load instruction covers the window
+        // of 64 / kWinL chains (heavies: 64 / kWinH), kWinL consecutive list
+        // entries each, so that it touches few lines.
+        const int i0 = i, j0 = j;
+        {
+            double vw[kWinL], hw[kWinH];
+            int32_t vi[kWinL], hv[kWinH];
+#pragma unroll
+            for (int k = 0; k < kWinL; ++k) {
+                const int c = k * (64 / kWinL) + lane / kWinL, t = lane % kWinL;
+                const int q = min(__shfl(i0, c, 64) + t, __shfl(iLast, c, 64));
+                const bool load = __shfl((int)run, c, 64) != 0;
+                vw[k] = load ? a.Lw[q] : 0.0;
+                vi[k] = load ? a.Li[q] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kWinH; ++k) {
+                const int c = k * (64 / kWinH) + lane / kWinH, t = lane % kWinH;
+                const int q = min(__shfl(j0, c, 64) + t, __shfl(jUpper, c, 64));
+                const bool load = __shfl((int)run, c, 64) != 0;
+                hw[k] = load ? a.Hw[q] : 0.0;
+                hv[k] = load ? a.Hi[q] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kWinL; ++k) {
+                const int c = k * (64 / kWinL) + lane / kWinL, t = lane % kWinL;
+                wl[t][c] = vw[k];
+                il[t][c] = vi[k];
+            }
+#pragma unroll
+            for (int k = 0; k < kWinH; ++k) {
+                const int c = k * (64 / kWinH) + lane / kWinH, t = lane % kWinH;
+                wh[t][c] = hw[k];
+                ih[t][c] = hv[k];
+            }
+        }
+        wave_lds_sync();
+        while (run) {
+            const bool idone = i >= iUpper, jdone = j >= jUpper;
+            if (idone && jdone) {
+                run = false;
+                break;
+            }
+            // A heavy step when the bucket is full (or no light is left) and
+            // a heavy remains; a light step otherwise (pack_chain's rule).
+            const bool ex = idone || (w <= thr && !jdone);
+            const int sl = i - i0, sh = j - j0;
+            if (ex ? sh + 1 >= kWinH : sl >= kWinL) break;  // past the window: restage
+            // sh < kWinH always (a heavy step needs slot sh + 1); sl can
+            // reach kWinL on a heavy step, whose light is not used.
+            const int tl = min(sl, kWinL - 1), th1 = min(sh + 1, kWinH - 1);
+            const double lw = wl[tl][lane], hw1 = wh[th1][lane];
+            const int hj = ih[sh][lane], hi1 = ih[th1][lane];
+            if (ex) {
+                ah[sh][lane] = hi1;
+                wh[sh][lane] = w;
+            } else {
+                al[tl][lane] = hj;
+            }
+            w += (ex ? hw1 : lw) - W_N;
+            j += ex ? 1 : 0;
+            i += ex ? 0 : 1;
+        }
+        wave_lds_sync();
+        // The round's rows (lights i0 .. i - 1, heavies j0 .. j - 1 of each
+        // chain), written a window at a time: one store instruction covers
+        // the consecutive light rows of 64 / kWinL chains (heavy rows: of
+        // 64 / kWinH chains), whose items rise along the lists, so that it
+        // touches few table lines.
+        const int nl = i - i0, nh = j - j0;
+#pragma unroll
+        for (int k = 0; k < kWinL; ++k) {
+            const int c = k * (64 / kWinL) + lane / kWinL, t = lane % kWinL;
+            const int it = il[t][c];
+            if (t < __shfl(nl, c, 64) && (uint32_t)it < (uint32_t)a.n) store_row<AOS>(a, it, wl[t][c], al[t][c]);
+        }
+#pragma unroll
+        for (int k = 0; k < kWinH; ++k) {
+            const int c = k * (64 / kWinH) + lane / kWinH, t = lane % kWinH;
+            const int it = ih[t][c];
+            if (t < __shfl(nh, c, 64) && (uint32_t)it < (uint32_t)a.n) store_row<AOS>(a, it, wh[t][c], ah[t][c]);
+        }
+        wave_lds_sync();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers.
 static inline unsigned blocks_for(int64_t items, int per) { return (unsigned)((items + per - 1) / per); }
@@ -234,6 +364,9 @@ hipError_t launch_pack_blocks(const BuildArgs& a, hipStream_t s, int64_t b0, int
             if (a.rows) hipLaunchKernelGGL((pack_basic<true, true, double>), g, b, 0, s, a, b0);
             else hipLaunchKernelGGL((pack_basic<false, true, double>), g, b, 0, s, a, b0);
         }
+    } else if (pack_windowed_ok(a.n, a.p_req, a.pack_method) && !a.lean_lights) {
+        if (a.rows) hipLaunchKernelGGL(pack_windowed<true>, g, b, 0, s, a, b0);
+        else hipLaunchKernelGGL(pack_windowed<false>, g, b, 0, s, a, b0);
     } else {
         if (a.rows) hipLaunchKernelGGL(pack_staged<true>, g, b, lds, s, a, b0);
         else hipLaunchKernelGGL(pack_staged<false>, g, b, lds, s, a, b0);

@@ -288,3 +288,26 @@ def test_pack_methods_identical(gpu, oracle, pack, n, dist, psa, p, split, layou
     t.sync()
     assert_same_build(t, ref, n, check_lists=False)
     t.close()
+
+
+@pytest.mark.parametrize("n,dist,psa,p,layout,dtype", [
+    (1 << 20, "pareto", True, 2000, "aos", "f64"),
+    (1 << 20, "pareto", False, 1000, "soa", "f64"),
+    (1_000_003, "heavy_mix", True, 300, "aos", "f32"),
+    (300_007, "unit_uniform", True, 100, "soa", "f64"),
+    (200_003, "sine", False, 2, "aos", "f64"),
+])
+def test_long_chain_pack_matches_oracle(gpu, oracle, n, dist, psa, p, layout, dtype):
+    """Chains of more than 256 items (n / p > 256) run the windowed pack: each
+    lane steps through windows of its lists staged in LDS.  Rows, lists and
+    splits equal the oracle's, on a fresh build and on a rebuild."""
+    w = atg.generate_weights(dist, n, seed=29, param=1.1 if dist == "pareto" else 0.9)
+    if dtype == "f32":
+        w = w.astype(np.float32)
+    t = gpu_build(w, dtype=dtype, psa_plus=psa, num_splits=p, layout=layout)
+    ref = oracle.build(w.astype(np.float64), psa_plus=psa, num_splits=p)
+    assert_same_build(t, ref, n)
+    t.build()
+    t.sync()
+    assert_same_build(t, ref, n)
+    t.close()
