@@ -91,3 +91,11 @@ def test_noalloc_pack_methods(gpu, pack, psa):
     torch = gpu
     check_against_build(torch, 100_003, "pareto", "aos", "f64", "bisect", 5, pack=pack, psa_plus=psa)
     check_against_build(torch, 5000, "sine", "soa", "f32", "learned", 6, pack=pack, psa_plus=psa)
+
+
+def test_noalloc_long_chains(gpu):
+    """Chains of more than 256 items (the windowed pack, static LDS only):
+    two calls in a row, the second smaller."""
+    torch = gpu
+    check_against_build(torch, 1 << 20, "pareto", "aos", "f64", "bisect", 8, num_splits=1000)
+    check_against_build(torch, 300_007, "unit_uniform", "soa", "f32", "learned", 9, num_splits=100)
