@@ -93,6 +93,44 @@ __device__ __forceinline__ double wave_scan(double v) {
     v = v + dpp_d<0x143, 0xC, false>(v);
     return v;
 }
+// Lane l reads lane l ^ XOR (XOR < 32) of a double: ds_swizzle in bit mode
+// (and-mask 0x1F, xor-mask XOR, lanes in groups of 32).
+template <int XOR>
+__device__ __forceinline__ double swizzle_xor_d(double v) {
+    constexpr int kPattern = 0x1F | (XOR << 10);
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)u, kPattern);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(u >> 32), kPattern);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The pairwise-tree sums of eight vectors at once, each with the association
+// of lane 63 of wave_scan (level b adds the two sibling blocks of 2^(b-1)
+// lanes).  A butterfly over the lane bits: in levels 1-3 bit b-1 of the lane
+// picks which of two vectors (or pairs, or quads) a lane keeps summing, so
+// one register carries all eight from level 3 on.  Lane l returns the sum of
+// vector l & 7.
+__device__ __forceinline__ double wave_tree_sum8(const double (&v)[8], int lane) {
+    const bool b0 = lane & 1, b1 = (lane >> 1) & 1, b2 = (lane >> 2) & 1;
+    double r1[4], r2[2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {  // lane l ^ 1 by quad_perm [1,0,3,2]
+        const double x = b0 ? v[2 * q + 1] : v[2 * q], y = b0 ? v[2 * q] : v[2 * q + 1];
+        r1[q] = x + dpp_d<0xB1, 0xF, false>(y);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {  // lane l ^ 2 by quad_perm [2,3,0,1]
+        const double x = b1 ? r1[2 * q + 1] : r1[2 * q], y = b1 ? r1[2 * q] : r1[2 * q + 1];
+        r2[q] = x + dpp_d<0x4E, 0xF, false>(y);
+    }
+    const double x = b2 ? r2[1] : r2[0], y = b2 ? r2[0] : r2[1];
+    double r = x + swizzle_xor_d<4>(y);
+    r = r + dpp_d<0x128, 0xF, false>(r);  // lane l ^ 8: row_ror:8
+    r = r + swizzle_xor_d<16>(r);
+    r = r + shfl_xor_d(r, 32);
+    return r;
+}
+
 // Value of the previous lane (0 for lane 0): wave_shr:1.
 __device__ __forceinline__ double wave_shr1(double v) { return dpp_d<0x138, 0xF, true>(v); }
 

@@ -344,8 +344,9 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
 
     // One pass per list: stage the leftovers (weight, item) in rank order
     // (lights at [start, start + mL), heavies at [start + mL, start + mL + mH))
-    // and form the canonical local-scan aggregate (chunks of 64, wave scan,
-    // chunk sums added left to right), PLACE_GROUP chunks at a time.
+    // and form the canonical local-scan aggregate (chunks of 64, each summed
+    // by the pairwise tree of the wave scan's last lane, chunk sums added left
+    // to right), PLACE_GROUP chunks at a time.
     double agg[2] = {0.0, 0.0};
     for (int hv = 0; hv < 2; ++hv) {
         const bool heavy = hv == 1;
@@ -375,9 +376,12 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
                     __builtin_nontemporal_store((int32_t)(start + it), &a.stage_i[start + rank0 + r]);
                 }
             }
+            // The chunk sums of the group (lane u: chunk u), added left to right.
+            static_assert(PLACE_GROUP == 8, "wave_tree_sum8 sums eight chunks");
+            const double t8 = wave_tree_sum8(v, lane);
 #pragma unroll
             for (int u = 0; u < PLACE_GROUP; ++u)
-                if (g0 + 64 * u < m) agg[hv] = agg[hv] + readlane_d(wave_scan(v[u]), 63);
+                if (g0 + 64 * u < m) agg[hv] = agg[hv] + readlane_d(t8, u);
         }
     }
     const double aggL = agg[0], aggH = agg[1];
