@@ -545,7 +545,8 @@ __device__ __forceinline__ void sum_finish_block(const BuildArgs& a) {
     int64_t count = a.G;
     while (true) {
         const int64_t groups = (count + 63) / 64;
-        // Wave w reduces groups w, w + nw, ...; eight groups' loads in flight.
+        // Wave w reduces groups w, w + nw, ...: eight groups' loads in
+        // flight, their pairwise-tree sums at once (lane u < 8: group u).
         for (int64_t g0 = wave; g0 < groups; g0 += 8 * nw) {
             double v[8];
 #pragma unroll
@@ -553,14 +554,9 @@ __device__ __forceinline__ void sum_finish_block(const BuildArgs& a) {
                 const int64_t i = (g0 + u * nw) * 64 + lane;
                 v[u] = (i < count) ? src[i] : 0.0;
             }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int64_t g = g0 + u * nw;
-                if (g < groups) {
-                    const double r = wave_tree_sum(v[u]);
-                    if (lane == 0) bufs[cur][g] = r;
-                }
-            }
+            const double r = wave_tree_sum8(v, lane);
+            const int64_t g = g0 + lane * nw;
+            if (lane < 8 && g < groups) bufs[cur][g] = r;
         }
         __syncthreads();
         src = bufs[cur];
