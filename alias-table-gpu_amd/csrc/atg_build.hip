@@ -315,7 +315,7 @@ hipError_t launch_partition_place(const BuildArgs& a, hipStream_t s) {
     if (a.lean_lights) hipLaunchKernelGGL(partition_place<true>, dim3(blocks_for(a.G, 4)), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(partition_place<false>, dim3(blocks_for(a.G, 4)), dim3(256), 0, s, a);
     if (a.want_cert)
-        hipLaunchKernelGGL(cert_reduce, dim3(blocks_for(a.G, 256 * 16)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(cert_reduce, dim3(blocks_for(a.G, 256)), dim3(256), 0, s, a);  // a section per thread
     return hipGetLastError();
 }
 
