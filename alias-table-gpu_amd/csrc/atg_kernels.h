@@ -293,31 +293,35 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
             // new indices (within the chain's ranges) for the following step.
             // The loop is one straight run of instructions besides its exit.
             // Lights and heavies share one LDS array (heavy j at len-1-j).
-            int i = iLo, j = jLo;
-            double wc = L.Hw(j);
-            int hj = L.Hi(j);
+            // The next heavy j + 1 <= jHi <= nH sits at slot hn = len - 2 - j
+            // >= nL - 1 >= 0 (the chain has a light); past the chain's range
+            // it is read but never used.  The exit test is lane-mask logic.
+            int i = iLo, hn = len - 2 - jLo;
+            double wc = L.Hw(jLo);
+            int hj = L.Hi(jLo);
             double lw = L.Lw(i);
             int li = L.Li(i);
-            int q = min(j + 1, jHi - 1);
-            double hw1 = L.Hw(q);
-            int hi1 = L.Hi(q);
+            double hw1 = L.w[hn];
+            int hi1 = L.idx[hn];
             const double thr = W_N + ATG_EPSILON;
+            const int iEnd = iHi - 1, hnEnd = len - 1 - jHi;
             while (true) {
                 // Full bucket: the heavy's row, then the next heavy; else the
                 // light's row.  A step needs a further element on its side.
                 const bool ex = wc <= thr;
-                if ((ex ? jHi - 1 - j : iHi - 1 - i) <= 0) break;
+                const uint64_t mEx = __ballot(ex), mH = __ballot(hn <= hnEnd), mL = __ballot(i >= iEnd);
+                if (__builtin_amdgcn_inverse_ballot_w64((mEx & mH) | (~mEx & mL))) break;
                 store_row<AOS>(a, start + (ex ? hj : li), ex ? wc : lw, (int32_t)(start + (ex ? hi1 : hj)));
                 wc += (ex ? hw1 : lw) - W_N;
                 hj = ex ? hi1 : hj;
-                j += ex ? 1 : 0;
-                i += ex ? 0 : 1;
+                hn -= (int)ex;
+                i += (int)!ex;
                 lw = L.Lw(i);
                 li = L.Li(i);
-                q = min(j + 1, jHi - 1);
-                hw1 = L.Hw(q);
-                hi1 = L.Hi(q);
+                hw1 = L.w[hn];
+                hi1 = L.idx[hn];
             }
+            const int j = len - 2 - hn;
             // Remainder (PsaPlusNormal.cu:112-123): back to L when its
             // residual is light and a consumed light slot can hold it,
             // otherwise back to H in place.
