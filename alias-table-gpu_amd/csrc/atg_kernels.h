@@ -185,14 +185,16 @@ __device__ __forceinline__ void tree_block(const BuildArgs& a, int64_t q, int b0
 // chain hold the total).  The chain of rank r is the largest t with
 // off_t <= r, found by a binary search over the lanes; the U searches run
 // interleaved, so each step issues U lane reads before it waits.  All lanes
-// take part.
+// take part.  narrow: at most 32 chains, so lanes 32.. hold the total (> r)
+// and the search starts at step 16.
 template <int U>
-__device__ __forceinline__ void chain_positions(int r0, int off, int lo, int* pos) {
+__device__ __forceinline__ void chain_positions(int r0, int off, int lo, int* pos, bool narrow) {
     int t[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) t[u] = 0;
 #pragma unroll
     for (int step = 32; step >= 1; step >>= 1) {
+        if (step == 32 && narrow) continue;
         int o[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) o[u] = __shfl(off, t[u] + step, 64);
@@ -222,6 +224,43 @@ __device__ __forceinline__ void section_load(const BuildArgs& a, int64_t s, int 
         const int k = (u << 6) + lane;
         x[u] = (k < len) ? w[start + k] : T(0);
     }
+}
+
+// Leftover ranks g0 + 64 u + lane (u < U) of one class staged at base + rank
+// (weight, item) and their chunk sums added to agg left to right.
+template <int U>
+__device__ __forceinline__ void stage_chunks(const BuildArgs& a, SecLDS& L, bool heavy, bool identity, bool narrow,
+                                             int g0, int m, int off, int lo, int64_t start, int64_t base, int lane,
+                                             double& agg) {
+    int pos[U];
+    if (identity) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) pos[u] = g0 + 64 * u + lane;
+    } else {
+        chain_positions<U>(g0 + lane, off, lo, pos, narrow);
+    }
+    double v[PLACE_GROUP];
+#pragma unroll
+    for (int u = 0; u < PLACE_GROUP; ++u) v[u] = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int r = g0 + 64 * u + lane;
+        if (r < m) {
+            v[u] = heavy ? L.Hw(pos[u]) : L.Lw(pos[u]);
+            const int it = heavy ? L.Hi(pos[u]) : L.Li(pos[u]);
+            // Streaming stores: the staging is read once, by the
+            // placement pass (non-temporal row or list stores were
+            // slower: those lines are merged or re-read in L2).
+            __builtin_nontemporal_store(v[u], &a.stage_w[base + r]);
+            __builtin_nontemporal_store((int32_t)(start + it), &a.stage_i[base + r]);
+        }
+    }
+    // The chunk sums of the group (lane u: chunk u), added left to right.
+    static_assert(PLACE_GROUP == 8, "wave_tree_sum8 sums eight chunks");
+    const double t8 = wave_tree_sum8(v, lane);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (g0 + 64 * u < m) agg = agg + readlane_d(t8, u);
 }
 
 // Partition pass 1, section s by one wave, its weights in x (section_load):
@@ -350,42 +389,23 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
     // (lights at [start, start + mL), heavies at [start + mL, start + mL + mH))
     // and form the canonical local-scan aggregate (chunks of 64, each summed
     // by the pairwise tree of the wave scan's last lane, chunk sums added left
-    // to right), PLACE_GROUP chunks at a time.
+    // to right), up to PLACE_GROUP chunks at a time (the last group of a list
+    // takes 2, 4 or 8, as many as it holds).
+    const bool narrow = a.chains <= 32;
     double agg[2] = {0.0, 0.0};
     for (int hv = 0; hv < 2; ++hv) {
         const bool heavy = hv == 1;
         const int m = heavy ? mH : mL;
-        const int rank0 = heavy ? mL : 0;
+        const int64_t base = start + (heavy ? mL : 0);
         const int off = heavy ? offH : offL, lo = heavy ? jLo : iLo;
         for (int g0 = 0; g0 < m; g0 += 64 * PLACE_GROUP) {
-            double v[PLACE_GROUP];
-            int pos[PLACE_GROUP];
-            if (identity) {
-#pragma unroll
-                for (int u = 0; u < PLACE_GROUP; ++u) pos[u] = g0 + 64 * u + lane;
-            } else {
-                chain_positions<PLACE_GROUP>(g0 + lane, off, lo, pos);
-            }
-#pragma unroll
-            for (int u = 0; u < PLACE_GROUP; ++u) {
-                const int r = g0 + 64 * u + lane;
-                v[u] = 0.0;
-                if (r < m) {
-                    v[u] = heavy ? L.Hw(pos[u]) : L.Lw(pos[u]);
-                    const int it = heavy ? L.Hi(pos[u]) : L.Li(pos[u]);
-                    // Streaming stores: the staging is read once, by the
-                    // placement pass (non-temporal row or list stores were
-                    // slower: those lines are merged or re-read in L2).
-                    __builtin_nontemporal_store(v[u], &a.stage_w[start + rank0 + r]);
-                    __builtin_nontemporal_store((int32_t)(start + it), &a.stage_i[start + rank0 + r]);
-                }
-            }
-            // The chunk sums of the group (lane u: chunk u), added left to right.
-            static_assert(PLACE_GROUP == 8, "wave_tree_sum8 sums eight chunks");
-            const double t8 = wave_tree_sum8(v, lane);
-#pragma unroll
-            for (int u = 0; u < PLACE_GROUP; ++u)
-                if (g0 + 64 * u < m) agg[hv] = agg[hv] + readlane_d(t8, u);
+            const int nc = (m - g0 + 63) >> 6;
+            if (nc > 4)
+                stage_chunks<PLACE_GROUP>(a, L, heavy, identity, narrow, g0, m, off, lo, start, base, lane, agg[hv]);
+            else if (nc > 2)
+                stage_chunks<4>(a, L, heavy, identity, narrow, g0, m, off, lo, start, base, lane, agg[hv]);
+            else
+                stage_chunks<2>(a, L, heavy, identity, narrow, g0, m, off, lo, start, base, lane, agg[hv]);
         }
     }
     const double aggL = agg[0], aggH = agg[1];
