@@ -216,13 +216,16 @@ __device__ __forceinline__ void chain_positions(int r0, int off, int lo, int* po
 // conversion waits for its load); all loads in flight at once.
 template <typename T, int XU>
 __device__ __forceinline__ void section_load(const BuildArgs& a, int64_t s, int lane, T (&x)[XU]) {
-    const T* w = (const T*)a.weights;
-    const int64_t start = min(a.n, s * a.Ls);
-    const int len = (int)(min(a.n, (s + 1) * a.Ls) - start);
+    const T* w = (const T*)a.weights + min(a.n, s * a.Ls);
+    const int len = (int)(min(a.n, (s + 1) * a.Ls) - min(a.n, s * a.Ls));
+    if (len == 64 * XU) {
 #pragma unroll
-    for (int u = 0; u < XU; ++u) {
-        const int k = (u << 6) + lane;
-        x[u] = (k < len) ? w[start + k] : T(0);
+        for (int u = 0; u < XU; ++u) x[u] = w[(u << 6) + lane];
+    } else if (len > 0) {
+        // A short section: lanes past its end read its last item (never
+        // used), so no load needs its own exec mask.
+#pragma unroll
+        for (int u = 0; u < XU; ++u) x[u] = w[min((u << 6) + lane, len - 1)];
     }
 }
 
@@ -293,7 +296,21 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
 #pragma unroll
     for (int u = 0; u < XU; ++u) {
         const int k = (u << 6) + lane;
-        if ((u << 6) < len) {
+        if ((u << 6) + 64 <= len) {
+            // A full chunk, without branches: a light's slot is nL + (lights
+            // before the lane), a heavy's len - 1 - nH - (heavies before it).
+            const double xv = (double)x[u];
+            const bool light = xv <= W_N;
+            const uint64_t bl = __ballot(light);
+            const int before = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bl >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)bl, 0u));
+            const int slot = before + (light ? nL : len - 1 - nH - lane);
+            L.w[slot] = xv;
+            L.idx[slot] = (uint16_t)k;
+            const int c = __popcll(bl);
+            nL += c;
+            nH += 64 - c;
+        } else if ((u << 6) < len) {
             const bool valid = k < len;
             const double xv = (double)x[u];
             const bool light = valid && xv <= W_N, heavy = valid && !(xv <= W_N);
