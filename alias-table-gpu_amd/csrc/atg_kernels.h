@@ -396,9 +396,16 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
         iHi = nL;
         jHi = nH;
     }
+    // Both classes' leftover counts in one scan (each total is at most
+    // PSA_MAX_LEN < 2^16): offsets from the exclusive scan, totals from the
+    // last lane.
+    static_assert(PSA_MAX_LEN < 65536, "16-bit count fields");
     const int cl = iHi - iLo, ch = jHi - jLo;
-    const int offL = wave_excl_scan_int(cl), offH = wave_excl_scan_int(ch);
-    const int mL = wave_sum_int(cl), mH = wave_sum_int(ch);
+    const uint32_t cp = (uint32_t)cl | ((uint32_t)ch << 16);
+    const uint32_t incl = wave_incl_scan_u32(cp), excl = incl - cp;
+    const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    const int offL = (int)(excl & 0xFFFFu), offH = (int)(excl >> 16);
+    const int mL = (int)(tot & 0xFFFFu), mH = (int)(tot >> 16);
     const bool identity = !a.psa_plus;
     wave_lds_sync();
 
