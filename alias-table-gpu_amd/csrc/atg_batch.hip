@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void seg_place(SegArgs g, SegMap m) {
     seg_locate(m, t, blk);
     const BuildArgs& a = g.tables[t];
     if (a.st->error) return;
-    const int64_t s = blk * 4 + (threadIdx.x >> 6);
+    const int64_t s = blk * 4 + wave_in_block();
     if (s >= a.G) return;
     place_section<false>(a, s, (int)(threadIdx.x & 63));
 }

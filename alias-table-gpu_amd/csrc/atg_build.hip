@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void cert_reduce(BuildArgs a) {
 template <bool LEAN>
 __global__ __launch_bounds__(256) void partition_place(BuildArgs a) {
     if (a.st->error) return;
-    const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t s = (int64_t)blockIdx.x * 4 + wave_in_block();
     if (s >= a.G) return;
     place_section<LEAN>(a, s, (int)(threadIdx.x & 63));
 }

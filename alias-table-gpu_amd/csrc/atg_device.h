@@ -60,6 +60,9 @@ __device__ __forceinline__ uint64_t scale64(uint64_t x, uint64_t n) { return __u
 // ---------------------------------------------------------------------------
 // Wavefront primitives (wave64).
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+// The wave's index in its block, as a scalar: values derived from it (a
+// section per wave) stay uniform, so tests on them are scalar branches.
+__device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, ATG_WAVE); }
 __device__ __forceinline__ double shfl_up_d(double v, int d) { return __shfl_up(v, d, ATG_WAVE); }

@@ -31,19 +31,36 @@ __device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t
     bool bad = false;
     for (int64_t k0 = start; k0 < end; k0 += 32 * 64) {
         // 32 items per lane in flight, kept raw: a conversion next to each
-        // load would serialise the loads.
+        // load would serialise the loads.  Chunks past the end are skipped;
+        // in the last one, items past the end read the last item (no exec
+        // masks around the loads) and are not added.
         T xr[32];
+        if (k0 + 32 * 64 <= end) {
 #pragma unroll
-        for (int u = 0; u < 32; ++u) {
-            const int64_t k = k0 + u * 64 + lane;
-            xr[u] = (k < end) ? __builtin_nontemporal_load(&w[k]) : T(0);
-        }
+            for (int u = 0; u < 32; ++u) xr[u] = __builtin_nontemporal_load(&w[k0 + u * 64 + lane]);
 #pragma unroll
-        for (int u = 0; u < 32; ++u) {
-            if (k0 + u * 64 + lane < end) {
+            for (int u = 0; u < 32; ++u) {
                 const double x = (double)xr[u];
                 bad |= !(x >= 0.0) || (x == __builtin_inf());
                 acc = acc + x;
+            }
+        } else {
+            // The section's tail: a scalar base and 32-bit lane offsets.
+            const char* wb = reinterpret_cast<const char*>(w + k0);
+            const int lim = (int)(end - 1 - k0);
+#pragma unroll
+            for (int u = 0; u < 32; ++u)
+                xr[u] = (u * 64 <= lim)
+                            ? __builtin_nontemporal_load(reinterpret_cast<const T*>(
+                                  wb + (uint32_t)min(u * 64 + lane, lim) * (uint32_t)sizeof(T)))
+                            : T(0);
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                if (k0 + u * 64 + lane < end) {
+                    const double x = (double)xr[u];
+                    bad |= !(x >= 0.0) || (x == __builtin_inf());
+                    acc = acc + x;
+                }
             }
         }
     }
@@ -479,16 +496,34 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
         for (int g0 = 0; g0 < m; g0 += 64 * PLACE_CHUNKS) {
             double v[PLACE_CHUNKS], inc[PLACE_CHUNKS];
             int32_t item[PLACE_CHUNKS];
+            // Chunks past the list are skipped; in the last one, ranks past
+            // the list read its last entry (no exec masks around the loads)
+            // and count as 0.
+            if (g0 + 64 * PLACE_CHUNKS <= m) {
 #pragma unroll
-            for (int u = 0; u < PLACE_CHUNKS; ++u) {
-                const int r = g0 + 64 * u + lane;
-                v[u] = 0.0;
-                item[u] = 0;
-                if (r < m) {
-                    v[u] = __builtin_nontemporal_load(&a.stage_w[src + r]);
-                    if (full) item[u] = __builtin_nontemporal_load(&a.stage_i[src + r]);
+                for (int u = 0; u < PLACE_CHUNKS; ++u) {
+                    v[u] = __builtin_nontemporal_load(&a.stage_w[src + g0 + 64 * u + lane]);
+                    item[u] = full ? __builtin_nontemporal_load(&a.stage_i[src + g0 + 64 * u + lane]) : 0;
+                }
+            } else {
+                // The list's last group: a scalar base and 32-bit lane offsets.
+                const char* bw = reinterpret_cast<const char*>(a.stage_w + src + g0);
+                const char* bi = reinterpret_cast<const char*>(a.stage_i + src + g0);
+                const int lim = m - 1 - g0;
+#pragma unroll
+                for (int u = 0; u < PLACE_CHUNKS; ++u) {
+                    const uint32_t rc = (uint32_t)min(64 * u + lane, lim);
+                    v[u] = 0.0;
+                    item[u] = 0;
+                    if (64 * u <= lim) {
+                        v[u] = __builtin_nontemporal_load(reinterpret_cast<const double*>(bw + rc * 8u));
+                        if (full) item[u] = __builtin_nontemporal_load(reinterpret_cast<const int32_t*>(bi + rc * 4u));
+                    }
                 }
             }
+#pragma unroll
+            for (int u = 0; u < PLACE_CHUNKS; ++u)
+                if (g0 + 64 * u + lane >= m) v[u] = 0.0;
 #pragma unroll
             for (int u = 0; u < PLACE_CHUNKS; ++u) inc[u] = wave_scan(v[u]);
 #pragma unroll
@@ -571,7 +606,7 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
 // their arrival counters for the partition.
 template <typename T>
 __device__ __forceinline__ void sum_sections_block(const BuildArgs& a, int64_t blk, int64_t nblk) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_in_block();
     const T* w = (const T*)a.weights;
     const int64_t s = blk * (blockDim.x >> 6) + wave;
     if (s < a.G) {
@@ -1024,7 +1059,7 @@ __device__ __forceinline__ void degenerate_block(const BuildArgs& a, int64_t blk
         // The lights from the staging, one wave per section; then the heavies.
         const int lane = threadIdx.x & 63;
         const int64_t wpb = blockDim.x >> 6;
-        for (int64_t s = blk * wpb + (threadIdx.x >> 6); s < a.G; s += nblk * wpb) {
+        for (int64_t s = blk * wpb + wave_in_block(); s < a.G; s += nblk * wpb) {
             const int64_t base = min(a.n, s * a.Ls);
             const int64_t m = (int64_t)a.sec_loff[s + 1] - (int64_t)a.sec_loff[s];
             for (int64_t r = lane; r < m; r += 64) {
@@ -1049,7 +1084,7 @@ __device__ __forceinline__ void lean_lights_fill_block(const BuildArgs& a, int64
     if (a.st->error) return;
     const int lane = threadIdx.x & 63;
     const int64_t wpb = blockDim.x >> 6;
-    for (int64_t s = blk * wpb + (threadIdx.x >> 6); s < a.G; s += nblk * wpb) {
+    for (int64_t s = blk * wpb + wave_in_block(); s < a.G; s += nblk * wpb) {
         const int64_t base = min(a.n, s * a.Ls), l0 = a.sec_loff[s];
         const int64_t m = (int64_t)a.sec_loff[s + 1] - l0;
         for (int64_t r = lane; r < m; r += 64) {
