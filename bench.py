@@ -44,7 +44,7 @@ KERNEL_NAMES = {"sum": "sum_sections+sum_finish", "partition_local": "partition_
 # workload at the given shape; the file holds the commands and the counts.
 MEASURED_TRAFFIC = {
     (100_000_000, 2048, 24, 1_562_500, "partition_local"): (
-        2_296_993_000, "profiles/build_pareto1e8_s2048_c24_p1562500_hbm_traffic.md"),
+        2_296_954_000, "profiles/build_pareto1e8_s2048_c24_p1562500_hbm_traffic.md"),
 }
 
 
