@@ -4,7 +4,7 @@
 Headline workload: N = 1e8 FP64 weights i.i.d. Pareto(shape 1.1, scale 1) from
 Philox seed 42, PSA+ build (W sum, light/heavy partition with PSA+ and the
 fused leftover prefix sums, split, pack) at an explicit build shape (section
-2048, 24 chains, n/p = 64; the shape changes the table, so it is an input,
+2048, 16 chains, n/p = 64; the shape changes the table, so it is an input,
 chosen once with atg_autotune_shape), with the knobs that leave the table
 unchanged (split method, knots, split/pack interleave) tuned at run time.
 One step = one whole build over the resident weights.
@@ -43,8 +43,8 @@ KERNEL_NAMES = {"sum": "sum_sections+sum_finish", "partition_local": "partition_
 # (scaled by the counters of a device copy of known size) for the headline
 # workload at the given shape; the file holds the commands and the counts.
 MEASURED_TRAFFIC = {
-    (100_000_000, 2048, 24, 1_562_500, "partition_local"): (
-        2_296_954_000, "profiles/build_pareto1e8_s2048_c24_p1562500_hbm_traffic.md"),
+    (100_000_000, 2048, 16, 1_562_500, "partition_local"): (
+        2_312_409_000, "profiles/build_pareto1e8_s2048_c16_p1562500_hbm_traffic.md"),
 }
 
 
@@ -432,7 +432,7 @@ def main():
     ap.add_argument("--n", type=int, default=100_000_000)
     ap.add_argument("--samples", type=int, default=1_000_000_000)
     ap.add_argument("--section-size", type=int, default=2048, help="PSA+ section (the explicit build shape)")
-    ap.add_argument("--chains", type=int, default=24)
+    ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--items-per-split", type=int, default=64, help="p = N / this")
     ap.add_argument("--split", choices=["bisect", "learned"], default="bisect",
                     help="split method when the run-time tuner is off")
