@@ -202,16 +202,16 @@ __device__ __forceinline__ void tree_block(const BuildArgs& a, int64_t q, int b0
 // chain hold the total).  The chain of rank r is the largest t with
 // off_t <= r, found by a binary search over the lanes; the U searches run
 // interleaved, so each step issues U lane reads before it waits.  All lanes
-// take part.  narrow: at most 32 chains, so lanes 32.. hold the total (> r)
-// and the search starts at step 16.
+// take part.  first: the search's first step, the largest power of two
+// below the chain count (lanes from the chain count on hold the total, > r).
 template <int U>
-__device__ __forceinline__ void chain_positions(int r0, int off, int lo, int* pos, bool narrow) {
+__device__ __forceinline__ void chain_positions(int r0, int off, int lo, int* pos, int first) {
     int t[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) t[u] = 0;
 #pragma unroll
     for (int step = 32; step >= 1; step >>= 1) {
-        if (step == 32 && narrow) continue;
+        if (step > first) continue;
         int o[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) o[u] = __shfl(off, t[u] + step, 64);
@@ -249,7 +249,7 @@ __device__ __forceinline__ void section_load(const BuildArgs& a, int64_t s, int 
 // Leftover ranks g0 + 64 u + lane (u < U) of one class staged at base + rank
 // (weight, item) and their chunk sums added to agg left to right.
 template <int U>
-__device__ __forceinline__ void stage_chunks(const BuildArgs& a, SecLDS& L, bool heavy, bool identity, bool narrow,
+__device__ __forceinline__ void stage_chunks(const BuildArgs& a, SecLDS& L, bool heavy, bool identity, int first,
                                              int g0, int m, int off, int lo, int64_t start, int64_t base, int lane,
                                              double& agg) {
     int pos[U];
@@ -257,7 +257,7 @@ __device__ __forceinline__ void stage_chunks(const BuildArgs& a, SecLDS& L, bool
 #pragma unroll
         for (int u = 0; u < U; ++u) pos[u] = g0 + 64 * u + lane;
     } else {
-        chain_positions<U>(g0 + lane, off, lo, pos, narrow);
+        chain_positions<U>(g0 + lane, off, lo, pos, first);
     }
     double v[PLACE_GROUP];
 #pragma unroll
@@ -432,7 +432,10 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
     // by the pairwise tree of the wave scan's last lane, chunk sums added left
     // to right), up to PLACE_GROUP chunks at a time (the last group of a list
     // takes 2, 4 or 8, as many as it holds).
-    const bool narrow = a.chains <= 32;
+    // The rank search's first step: the largest power of two below the
+    // chain count (one chain: step 1, a search over lanes 0 and 1).
+    int first = 1;
+    while (2 * first < a.chains) first *= 2;
     double agg[2] = {0.0, 0.0};
     for (int hv = 0; hv < 2; ++hv) {
         const bool heavy = hv == 1;
@@ -442,11 +445,11 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
         for (int g0 = 0; g0 < m; g0 += 64 * PLACE_GROUP) {
             const int nc = (m - g0 + 63) >> 6;
             if (nc > 4)
-                stage_chunks<PLACE_GROUP>(a, L, heavy, identity, narrow, g0, m, off, lo, start, base, lane, agg[hv]);
+                stage_chunks<PLACE_GROUP>(a, L, heavy, identity, first, g0, m, off, lo, start, base, lane, agg[hv]);
             else if (nc > 2)
-                stage_chunks<4>(a, L, heavy, identity, narrow, g0, m, off, lo, start, base, lane, agg[hv]);
+                stage_chunks<4>(a, L, heavy, identity, first, g0, m, off, lo, start, base, lane, agg[hv]);
             else
-                stage_chunks<2>(a, L, heavy, identity, narrow, g0, m, off, lo, start, base, lane, agg[hv]);
+                stage_chunks<2>(a, L, heavy, identity, first, g0, m, off, lo, start, base, lane, agg[hv]);
         }
     }
     const double aggL = agg[0], aggH = agg[1];

@@ -65,6 +65,11 @@ CASES = [
     (1 << 20, "pareto", True, 16, 0, 0),
     (1 << 20, "pareto", False, 16, 0, 1400),
     (1 << 20, "heavy_mix", True, 32, 2048, 0),
+    # chain counts that are not powers of two (the leftover rank search
+    # starts at the largest power of two below the count)
+    (100003, "pareto", True, 1, 1024, 0),
+    (200003, "exponential", True, 3, 1500, 0),
+    (300007, "pareto", True, 17, 2048, 0),
 ]
 
 
