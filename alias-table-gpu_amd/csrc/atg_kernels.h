@@ -31,9 +31,11 @@ __device__ __forceinline__ double section_sum(const T* w, int64_t start, int64_t
     bool bad = false;
     for (int64_t k0 = start; k0 < end; k0 += 32 * 64) {
         // 32 items per lane in flight, kept raw: a conversion next to each
-        // load would serialise the loads.  Chunks past the end are skipped;
-        // in the last one, items past the end read the last item (no exec
-        // masks around the loads) and are not added.
+        // load would serialise the loads.  start and end are scalars (the
+        // caller's section index is), so the tests on k0 are scalar
+        // branches.  Chunks past the end are skipped; in the last one, items
+        // past the end read the last item (no exec masks around the loads)
+        // and are not added.
         T xr[32];
         if (k0 + 32 * 64 <= end) {
 #pragma unroll
@@ -363,7 +365,7 @@ __device__ __forceinline__ void psa_section(const BuildArgs& a, int64_t s, T (&x
             // Registers hold the chain's current heavy (residual wc, item hj),
             // the next heavy (hw1, hi1) and the current light (lw, li); a step
             // stores one row and reads the light and the next heavy at its
-            // new indices (within the chain's ranges) for the following step.
+            // new indices for the following step.
             // The loop is one straight run of instructions besides its exit.
             // Lights and heavies share one LDS array (heavy j at len-1-j).
             // The next heavy j + 1 <= jHi <= nH sits at slot hn = len - 2 - j
