@@ -80,9 +80,11 @@ constexpr int64_t kShortSectionMax = 1024;
 // placing them.
 #define PLACE_GROUP 8
 // Chunks of 64 leftovers the placement pass keeps in flight (a 2048-item
-// section has at most 32).  8 rather than 16: fewer registers, more waves,
-// the same loads in flight per SIMD (1% faster placement).
-#define PLACE_CHUNKS 8
+// section has at most 32): 4 with lean light lists, 8 when it writes every
+// list (fewer registers and more waves against more loads per wave; each the
+// faster of 4, 8 and 16 for its pass).
+template <bool LEAN>
+constexpr int place_chunks() { return LEAN ? 4 : 8; }
 
 struct SecLDS {
     double* w;      // [Ls]: lights from the front, heavies from the back
@@ -493,20 +495,21 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
         // Lean lists: the lights keep their items and weights in the staging.
         const bool full = heavy || !LEAN;
         double ext = heavy ? __builtin_inf() : 0.0;
-        // Groups of PLACE_CHUNKS chunks of 64: all loads of a group in flight
+        constexpr int PC = place_chunks<LEAN>();
+        // Groups of PC chunks of 64: all loads of a group in flight
         // together, the chunk scans independent, then the carries chained left
         // to right (the same operations in the same order as one chunk at a
         // time).
         double carry = 0.0;
-        for (int g0 = 0; g0 < m; g0 += 64 * PLACE_CHUNKS) {
-            double v[PLACE_CHUNKS], inc[PLACE_CHUNKS];
-            int32_t item[PLACE_CHUNKS];
+        for (int g0 = 0; g0 < m; g0 += 64 * PC) {
+            double v[PC], inc[PC];
+            int32_t item[PC];
             // Chunks past the list are skipped; in the last one, ranks past
             // the list read its last entry (no exec masks around the loads)
             // and count as 0.
-            if (g0 + 64 * PLACE_CHUNKS <= m) {
+            if (g0 + 64 * PC <= m) {
 #pragma unroll
-                for (int u = 0; u < PLACE_CHUNKS; ++u) {
+                for (int u = 0; u < PC; ++u) {
                     v[u] = __builtin_nontemporal_load(&a.stage_w[src + g0 + 64 * u + lane]);
                     item[u] = full ? __builtin_nontemporal_load(&a.stage_i[src + g0 + 64 * u + lane]) : 0;
                 }
@@ -516,7 +519,7 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
                 const char* bi = reinterpret_cast<const char*>(a.stage_i + src + g0);
                 const int lim = m - 1 - g0;
 #pragma unroll
-                for (int u = 0; u < PLACE_CHUNKS; ++u) {
+                for (int u = 0; u < PC; ++u) {
                     const uint32_t rc = (uint32_t)min(64 * u + lane, lim);
                     v[u] = 0.0;
                     item[u] = 0;
@@ -527,12 +530,12 @@ __device__ __forceinline__ void place_section(const BuildArgs& a, int64_t s, int
                 }
             }
 #pragma unroll
-            for (int u = 0; u < PLACE_CHUNKS; ++u)
+            for (int u = 0; u < PC; ++u)
                 if (g0 + 64 * u + lane >= m) v[u] = 0.0;
 #pragma unroll
-            for (int u = 0; u < PLACE_CHUNKS; ++u) inc[u] = wave_scan(v[u]);
+            for (int u = 0; u < PC; ++u) inc[u] = wave_scan(v[u]);
 #pragma unroll
-            for (int u = 0; u < PLACE_CHUNKS; ++u) {
+            for (int u = 0; u < PC; ++u) {
                 const int r0 = g0 + 64 * u;
                 if (r0 < m) {
                     const int r = r0 + lane;
